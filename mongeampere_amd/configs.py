@@ -1,0 +1,215 @@
+"""Seeded synthetic inputs for the five benchmark/parity workloads.
+
+Every generator draws from ``numpy.random.Generator(PCG64(seed))`` with
+seed = workload number, in the order: density parameters -> vertex values ->
+Diracs -> target masses.  Grid meshes place vertex (i, j) at
+(i/(n-1), j/(n-1)) and split each square along its (i,j)-(i+1,j+1) diagonal,
+giving counter-clockwise triangles.  Densities are P1 (values at vertices)
+normalised to total mass 1 by their exact P1 integral.
+
+The reference builds its densities from images (``include/MA/functions.hpp:82-120``);
+``image_to_pl`` here restates that routine (grid on [-1,1]^2, value
+pixel/255 + 1e-3) for callers that have image data.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class Mesh:
+    vxy: np.ndarray   # (V, 2) float64
+    vval: np.ndarray  # (V,) float64, P1 density values at the vertices
+    tri: np.ndarray   # (T, 3) int32, counter-clockwise
+
+    @property
+    def nv(self):
+        return self.vxy.shape[0]
+
+    @property
+    def nt(self):
+        return self.tri.shape[0]
+
+    def total_mass(self):
+        return p1_mass(self.vxy, self.vval, self.tri)
+
+
+@dataclass
+class Problem:
+    name: str
+    mesh: Mesh
+    X: np.ndarray    # (N, 2) float64
+    nu: np.ndarray   # (N,) float64 target masses
+    grid_n: int = 0  # vertices per side for grid meshes (0 otherwise)
+
+
+def grid_mesh(n, x0=0.0, y0=0.0, x1=1.0, y1=1.0):
+    """n x n vertex grid on [x0,x1]x[y0,y1]; 2 (n-1)^2 ccw triangles."""
+    i = np.arange(n)
+    xs = x0 + (x1 - x0) * i / (n - 1)
+    ys = y0 + (y1 - y0) * i / (n - 1)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")  # vertex (i,j) -> index i*n + j
+    vxy = np.stack([X.ravel(), Y.ravel()], axis=1)
+    a = (i[:-1, None] * n + i[None, :-1]).ravel()      # (i, j)
+    b = a + n                                          # (i+1, j)
+    c = a + n + 1                                      # (i+1, j+1)
+    d = a + 1                                          # (i, j+1)
+    t1 = np.stack([a, b, c], axis=1)
+    t2 = np.stack([a, c, d], axis=1)
+    tri = np.empty((2 * a.size, 3), np.int32)
+    tri[0::2] = t1
+    tri[1::2] = t2
+    return vxy, tri
+
+
+def p1_mass(vxy, vval, tri):
+    p = vxy[tri]
+    ar = 0.5 * ((p[:, 1, 0] - p[:, 0, 0]) * (p[:, 2, 1] - p[:, 0, 1])
+                - (p[:, 2, 0] - p[:, 0, 0]) * (p[:, 1, 1] - p[:, 0, 1]))
+    return float(np.sum(np.abs(ar) * vval[tri].mean(axis=1)))
+
+
+def gaussian_mixture(xy, base, centers, sigmas, amps):
+    r = np.full(xy.shape[0], float(base))
+    for c, s, a in zip(centers, sigmas, amps):
+        d2 = ((xy - c) ** 2).sum(axis=1)
+        r += a * np.exp(-d2 / (2.0 * s * s))
+    return r
+
+
+def grid_p1_eval(n, vval, pts):
+    """Evaluate the P1 interpolant of a unit-square grid mesh at points."""
+    h = 1.0 / (n - 1)
+    u = np.clip(pts[:, 0] / h, 0, n - 1 - 1e-12)
+    v = np.clip(pts[:, 1] / h, 0, n - 1 - 1e-12)
+    i = np.floor(u).astype(np.int64)
+    j = np.floor(v).astype(np.int64)
+    fu = u - i
+    fv = v - j
+    f00 = vval[i * n + j]
+    f10 = vval[(i + 1) * n + j]
+    f11 = vval[(i + 1) * n + j + 1]
+    f01 = vval[i * n + j + 1]
+    lower = fu >= fv  # triangle (i,j),(i+1,j),(i+1,j+1)
+    return np.where(lower,
+                    f00 + fu * (f10 - f00) + fv * (f11 - f10),
+                    f00 + fv * (f01 - f00) + fu * (f11 - f01))
+
+
+def _normalise(mesh):
+    tot = mesh.total_mass()
+    mesh.vval = mesh.vval / tot
+    return mesh
+
+
+def config1(N=1000, n=32):
+    """Plain FP64, CPU-runnable: 32x32 grid, vertex values U[0.5,1.5], N=1000
+    uniform Diracs, nu = 1/N."""
+    rng = np.random.Generator(np.random.PCG64(1))
+    vxy, tri = grid_mesh(n)
+    vval = rng.uniform(0.5, 1.5, size=vxy.shape[0])
+    mesh = _normalise(Mesh(vxy, vval, tri))
+    X = rng.uniform(0.0, 1.0, size=(N, 2))
+    nu = np.full(N, 1.0 / N)
+    return Problem("C1", mesh, X, nu, n)
+
+
+def config2(N=100_000, n=256):
+    """FP64 GPU vs CPU: 256x256 grid, rho = 0.1 + 8 Gaussians, N=100k uniform."""
+    rng = np.random.Generator(np.random.PCG64(2))
+    K = 8
+    centers = rng.uniform(0.2, 0.8, size=(K, 2))
+    sigmas = rng.uniform(0.03, 0.15, size=K)
+    amps = rng.uniform(0.5, 1.0, size=K)
+    vxy, tri = grid_mesh(n)
+    vval = gaussian_mixture(vxy, 0.1, centers, sigmas, amps)
+    mesh = _normalise(Mesh(vxy, vval, tri))
+    X = rng.uniform(0.0, 1.0, size=(N, 2))
+    nu = np.full(N, 1.0 / N)
+    return Problem("C2", mesh, X, nu, n)
+
+
+def config3(N=1_000_000, n=1024):
+    """Mixed precision at scale (headline): 1024x1024 grid, rho = 0.05 + 32
+    Gaussians (sigma in [0.01,0.1]), N=1M Diracs by rejection sampling from the
+    P1 interpolant, lognormal(0,0.5) targets normalised to 1."""
+    rng = np.random.Generator(np.random.PCG64(3))
+    K = 32
+    centers = rng.uniform(0.1, 0.9, size=(K, 2))
+    sigmas = rng.uniform(0.01, 0.1, size=K)
+    amps = rng.uniform(0.5, 1.0, size=K)
+    vxy, tri = grid_mesh(n)
+    vval = gaussian_mixture(vxy, 0.05, centers, sigmas, amps)
+    mesh = _normalise(Mesh(vxy, vval, tri))
+    rmax = mesh.vval.max()
+    out = []
+    have = 0
+    while have < N:
+        m = max(2 * (N - have), 1024)
+        p = rng.uniform(0.0, 1.0, size=(m, 2))
+        u = rng.uniform(0.0, 1.0, size=m)
+        acc = p[u * rmax < grid_p1_eval(n, mesh.vval, p)]
+        out.append(acc)
+        have += acc.shape[0]
+    X = np.concatenate(out)[:N]
+    nu = rng.lognormal(0.0, 0.5, size=N)
+    nu /= nu.sum()
+    return Problem("C3", mesh, X, nu, n)
+
+
+def config4(N=200_000, n=512):
+    """Differentiation: 512x512 grid, rho = 1 + 2 exp(-|x-c|^2/(2*0.1^2)),
+    N=200k Diracs from 16 Gaussians (sigma=0.02), out-of-square draws
+    resampled, nu = 1/N."""
+    rng = np.random.Generator(np.random.PCG64(4))
+    vxy, tri = grid_mesh(n)
+    vval = gaussian_mixture(vxy, 1.0, [np.array([0.5, 0.5])], [0.1], [2.0])
+    mesh = _normalise(Mesh(vxy, vval, tri))
+    K = 16
+    centers = rng.uniform(0.1, 0.9, size=(K, 2))
+    out = []
+    have = 0
+    while have < N:
+        m = max(2 * (N - have), 1024)
+        k = rng.integers(0, K, size=m)
+        p = centers[k] + 0.02 * rng.standard_normal(size=(m, 2))
+        ok = (p >= 0.0).all(axis=1) & (p <= 1.0).all(axis=1)
+        out.append(p[ok])
+        have += int(ok.sum())
+    X = np.concatenate(out)[:N]
+    nu = np.full(N, 1.0 / N)
+    return Problem("C4", mesh, X, nu, n)
+
+
+def config(k, **kw):
+    return {1: config1, 2: config2, 3: config3, 4: config4}[k](**kw)
+
+
+def image_to_pl(image):
+    """Restates ``image_to_pl_function`` (``include/MA/functions.hpp:82-120``):
+    pixel (i, j) of an (n x m) image -> vertex (-1 + 2i/(n-1), -1 + 2j/(m-1))
+    with value image[m-j-1, i]/255 + 1e-3; returns (Mesh, total mass).
+    The reference triangulates the grid with CGAL's Delaunay triangulation,
+    whose diagonal choice on cocircular squares is CGAL-internal; here every
+    square is split along its (i,j)-(i+1,j+1) diagonal."""
+    img = np.asarray(image, dtype=np.float64)
+    m, n = img.shape  # rows = height, cols = width
+    xs = -1.0 + 2.0 * np.arange(n) / (n - 1)
+    ys = -1.0 + 2.0 * np.arange(m) / (m - 1)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    vxy = np.stack([X.ravel(), Y.ravel()], axis=1)
+    ii, jj = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")
+    vval = (img[m - jj - 1, ii] / 255.0 + 1e-3).ravel()
+    i = np.arange(n - 1)[:, None]
+    j = np.arange(m - 1)[None, :]
+    a = (i * m + j).ravel()
+    b = a + m
+    c = a + m + 1
+    d = a + 1
+    tri = np.empty((2 * a.size, 3), np.int32)
+    tri[0::2] = np.stack([a, b, c], axis=1)
+    tri[1::2] = np.stack([a, c, d], axis=1)
+    mesh = Mesh(vxy, vval, tri)
+    return mesh, mesh.total_mass()
