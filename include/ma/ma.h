@@ -1,0 +1,149 @@
+/*
+ * ma.h -- C ABI of the MI355X (gfx950) semi-discrete optimal transport engine.
+ *
+ * This is the drop-in boundary for the hot path of MongeAmpere++
+ * (power diagram x triangulation -> Kantorovich value / masses / Hessian ->
+ * damped Newton).  No templates, no CGAL/Eigen types: plain pointers and sizes.
+ *
+ * Conventions shared with the reference:
+ *   - power distance pi_i(x) = |x - y_i|^2 - w_i; a point tied between two
+ *     Diracs belongs to neither (include/MA/predicates.hpp:85-86);
+ *   - the density is P1 on a triangulation (values at vertices; per face a
+ *     linear function, include/MA/functions.hpp:55-80);
+ *   - Hessian entries H_ij = -int_{edge ij} rho ds / (2 |y_i - y_j|),
+ *     H_ii = -sum_j H_ij (include/MA/kantorovich.hpp:109-122).
+ *
+ * All arrays are in the caller's Dirac order (index i = row i of X).
+ * One engine = one device + one HIP stream; calls on one engine must not
+ * overlap; distinct engines are independent.
+ */
+#ifndef MA_H
+#define MA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ma_engine ma_engine;
+
+typedef enum {
+  MA_OK = 0,
+  MA_BAD_INPUT = 1,       /* wrong sizes, NaNs, non-triangles ...            */
+  MA_EMPTY_CELL = 2,      /* a Laguerre cell is empty (ma_stats.bad_index)    */
+  MA_SINGULAR = 3,        /* zero Hessian diagonal / disconnected cells       */
+  MA_NOT_CONVERGED = 4,   /* Newton reached maxiter                           */
+  MA_LINESEARCH_FAIL = 5, /* step length fell below alpha_min                 */
+  MA_OUT_OF_MEMORY = 6,
+  MA_HIP_ERROR = 7,
+  MA_CAPACITY = 8,        /* a fixed-size per-cell buffer overflowed          */
+  MA_NO_DATA = 9          /* mesh or Diracs not set yet                       */
+} ma_status;
+
+typedef enum { MA_FP64 = 0, MA_MIXED = 1 } ma_precision;
+typedef enum { MA_XY_INTERLEAVED = 0, MA_XY_COLMAJOR = 1 } ma_layout;
+typedef enum { MA_STOP_L2 = 0, MA_STOP_MAXREL = 1 } ma_stop_rule;
+
+typedef struct {
+  int precision;      /* ma_precision                                         */
+  int stop_rule;      /* MA_STOP_L2: ||m - nu||_2 < eps_g (reference rule,
+                         include/MA/optimal_transport.hpp:150);
+                         MA_STOP_MAXREL: max_i |m_i - nu_i| / nu_i <= tol     */
+  double eps_g;       /* default 1e-7                                          */
+  double tol;         /* max-relative tolerance for MA_STOP_MAXREL            */
+  int maxiter;        /* default 100; like the reference, up to maxiter + 1
+                         Newton steps run (optimal_transport.hpp:151)         */
+  double alpha_min;   /* backtracking floor (default 2^-30)                   */
+  double cg_tol;      /* relative residual of the linear solves (0 = auto)   */
+  int cg_maxiter;     /* default 20000                                        */
+  int verbose;        /* print one line per Newton step to stderr             */
+} ma_solve_opts;
+
+typedef struct {
+  int64_t niter;        /* Newton steps  (reference Statistics::niter)        */
+  int64_t neval;        /* Kantorovich evaluations (Statistics::neval)        */
+  int64_t cg_iter;      /* total linear-solver iterations                     */
+  int64_t bad_index;    /* Dirac index for MA_EMPTY_CELL                      */
+  double max_rel_err;   /* final max_i |m_i - nu_i| / nu_i (FP64)             */
+  double l2_err;        /* final ||m - nu||_2                                 */
+  double fval;          /* final value of F(w) = fval(w) - <nu, w>            */
+  double ms_total;      /* wall time of the solve                             */
+  double ms_eval;       /* time in evaluations                                */
+  double ms_linear;     /* time in linear solves                              */
+} ma_stats;
+
+/* Per-phase timing (ms, HIP events on the engine's stream) and counters of
+ * the last evaluation. */
+typedef struct {
+  double ms_total, ms_weights, ms_candidates, ms_cells, ms_assemble;
+  int64_t nnz;          /* Hessian nonzeros (incl. diagonal)                  */
+  int64_t nfrag;        /* non-empty (cell, triangle) fragments              */
+  int64_t ncand;        /* sum over bins of candidate owners                  */
+  int64_t max_poly;     /* largest cell polygon seen                          */
+  int64_t empty_cells;  /* Diracs with zero mass                              */
+} ma_eval_info;
+
+ma_status ma_create(int device, ma_engine **out);
+void ma_destroy(ma_engine *e);
+const char *ma_last_error(const ma_engine *e);
+
+/* Triangulated domain with P1 density values at the vertices.  Replaces the
+ * (T, Functions) pair of include/MA/kantorovich.hpp:37-38.  Triangles may be
+ * given in either orientation.  Data are copied. */
+ma_status ma_set_mesh(ma_engine *e, int64_t nv, const double *vxy, const double *vval,
+                      int64_t nt, const int32_t *tri);
+
+/* Dirac positions X (n x 2), replaces `const Matrix &X`
+ * (include/MA/kantorovich.hpp:39).  layout MA_XY_COLMAJOR matches Eigen's
+ * MatrixXd storage (all x, then all y).  Data are copied. */
+ma_status ma_set_diracs(ma_engine *e, int64_t n, const double *X, int layout);
+
+/* One evaluation of the Kantorovich functional at weights w (host array, n).
+ * Replaces MA::kantorovich (include/MA/kantorovich.hpp:35-141): *fval gets the
+ * value, g (host, n, may be NULL) the cell masses, *nnz the number of stored
+ * Hessian entries.  The Hessian stays on the device until ma_hessian_copy. */
+ma_status ma_kantorovich(ma_engine *e, const double *w, int precision, double *fval,
+                         double *g, int64_t *nnz);
+
+/* Same with device-resident weights and masses (d_g may be NULL). */
+ma_status ma_kantorovich_device(ma_engine *e, const double *d_w, int precision,
+                                double *fval, double *d_g, int64_t *nnz);
+
+/* Copies the last Hessian as CSR in the caller's Dirac order (rowptr n+1,
+ * col/val nnz; columns sorted; duplicates summed like setFromTriplets,
+ * include/MA/kantorovich.hpp:137-139). */
+ma_status ma_hessian_copy(ma_engine *e, int32_t *rowptr, int32_t *col, double *val);
+
+/* Damped Newton solve; replaces MA::ot_solve
+ * (include/MA/optimal_transport.hpp:89-193).  nu: target masses (n);
+ * w: in = initial guess (used when init_w != 0, otherwise zeros),
+ *    out = weights.  opts may be NULL (defaults). */
+ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
+                      const ma_solve_opts *opts, ma_stats *stats);
+
+/* Gauge-fixed Newton direction: solves H[:n-1,:n-1] d = g[:n-1], d[n-1] = 0
+ * for the Hessian of the last evaluation.  Replaces solve_laplacian_matrix
+ * (include/MA/optimal_transport.hpp:41-87).  *res gets ||H d - g|| over the
+ * first n-1 rows. */
+ma_status ma_solve_laplacian(ma_engine *e, const double *g, double *d, double rel_tol,
+                             int maxiter, double *res, int64_t *iters);
+
+/* Cell masses and first moments (int rho x, int rho y), replaces
+ * MA::first_moment (include/MA/lloyd.hpp:30-69).  mom is n x 2 interleaved. */
+ma_status ma_first_moment(ma_engine *e, const double *w, double *mass, double *mom);
+
+/* Adjoint of the transport cost sum_i int_{Lag_i} rho |x - y_i|^2 at weights w
+ * (normally the converged ones): dY (n x 2 interleaved) = d/dy_i with w held
+ * fixed (at optimal weights this equals the total derivative,
+ * 2 m_i (y_i - b_i)); dval (nv) = int (|x - y_i|^2 - w_i) phi_v(x) dx, the
+ * derivative w.r.t. vertex density values with w held fixed. */
+ma_status ma_adjoint(ma_engine *e, const double *w, double *dY, double *dval);
+
+ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info);
+void ma_default_solve_opts(ma_solve_opts *o);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MA_H */

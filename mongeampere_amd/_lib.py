@@ -1,0 +1,86 @@
+"""ctypes binding of the engine's C ABI (``include/ma/ma.h``).
+
+The shared library ``libmagpu.so`` is built in-tree by ``__graft_entry__.build()``.
+There is no fallback: if the library is missing, importing the engine raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmagpu.so")
+
+P = ctypes.c_void_p
+I32 = ctypes.c_int
+I64 = ctypes.c_int64
+F64 = ctypes.c_double
+
+STATUS = {
+    0: "MA_OK", 1: "MA_BAD_INPUT", 2: "MA_EMPTY_CELL", 3: "MA_SINGULAR",
+    4: "MA_NOT_CONVERGED", 5: "MA_LINESEARCH_FAIL", 6: "MA_OUT_OF_MEMORY",
+    7: "MA_HIP_ERROR", 8: "MA_CAPACITY", 9: "MA_NO_DATA",
+}
+MA_OK, MA_EMPTY_CELL, MA_NOT_CONVERGED, MA_LINESEARCH_FAIL = 0, 2, 4, 5
+
+
+class SolveOpts(ctypes.Structure):
+    _fields_ = [("precision", I32), ("stop_rule", I32), ("eps_g", F64), ("tol", F64),
+                ("maxiter", I32), ("alpha_min", F64), ("cg_tol", F64), ("cg_maxiter", I32),
+                ("verbose", I32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("niter", I64), ("neval", I64), ("cg_iter", I64), ("bad_index", I64),
+                ("max_rel_err", F64), ("l2_err", F64), ("fval", F64), ("ms_total", F64),
+                ("ms_eval", F64), ("ms_linear", F64)]
+
+
+class EvalInfo(ctypes.Structure):
+    _fields_ = [("ms_total", F64), ("ms_weights", F64), ("ms_candidates", F64),
+                ("ms_cells", F64), ("ms_assemble", F64), ("nnz", I64), ("nfrag", I64),
+                ("ncand", I64), ("max_poly", I64), ("empty_cells", I64)]
+
+
+# (name, restype, argtypes) of every exported entry point of include/ma/ma.h
+SIGNATURES = [
+    ("ma_create", I32, [I32, ctypes.POINTER(P)]),
+    ("ma_destroy", None, [P]),
+    ("ma_last_error", ctypes.c_char_p, [P]),
+    ("ma_set_mesh", I32, [P, I64, P, P, I64, P]),
+    ("ma_set_diracs", I32, [P, I64, P, I32]),
+    ("ma_kantorovich", I32, [P, P, I32, P, P, P]),
+    ("ma_kantorovich_device", I32, [P, P, I32, P, P, P]),
+    ("ma_hessian_copy", I32, [P, P, P, P]),
+    ("ma_ot_solve", I32, [P, P, P, I32, P, P]),
+    ("ma_solve_laplacian", I32, [P, P, P, F64, I32, P, P]),
+    ("ma_first_moment", I32, [P, P, P, P]),
+    ("ma_adjoint", I32, [P, P, P, P]),
+    ("ma_get_eval_info", I32, [P, P]),
+    ("ma_default_solve_opts", None, [P]),
+]
+
+_LIB = None
+
+
+def lib():
+    """Load ``libmagpu.so`` (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} not found: build the HIP engine first "
+                "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+class EngineError(RuntimeError):
+    def __init__(self, status, msg):
+        self.status = status
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")

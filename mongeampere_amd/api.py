@@ -1,0 +1,238 @@
+"""Python host interface mirroring the reference's operator API for the hot path.
+
+Reference entry points and their counterparts here:
+
+* ``MA::kantorovich(T, F, X, w, g, h) -> double``
+  (``include/MA/kantorovich.hpp:35-141``)      -> ``kantorovich(mesh, X, w)``
+* ``MA::solve_laplacian_matrix(h, g)``
+  (``include/MA/optimal_transport.hpp:41-87``) -> ``solve_laplacian_matrix(engine, g)``
+* ``MA::ot_solve(T, F, X, masses, x, eps_g, maxiter, verbose, stats)``
+  (``include/MA/optimal_transport.hpp:89-193``) -> ``ot_solve(mesh, X, masses, ...)``
+* ``MA::first_moment`` / ``MA::lloyd`` (``include/MA/lloyd.hpp:30-144``)
+                                                -> ``first_moment`` / ``lloyd``
+
+Everything runs on the GPU through ``libmagpu.so``; this module only moves
+arrays across the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import EngineError
+
+PRECISION = {"fp64": 0, "mixed": 1}
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class Statistics:
+    """Counterpart of ``MA::Statistics`` (``include/MA/optimal_transport.hpp:35-39``)
+    with the extra counters of the engine."""
+    niter: int = 0
+    neval: int = 0
+    cg_iter: int = 0
+    max_rel_err: float = float("nan")
+    l2_err: float = float("nan")
+    fval: float = float("nan")
+    ms_total: float = 0.0
+    ms_eval: float = 0.0
+    ms_linear: float = 0.0
+    status: str = "MA_OK"
+    bad_index: int = -1
+
+
+class Engine:
+    """One device + one HIP stream.  Mesh and Diracs are copied to the device
+    once; evaluations and solves then only move weights and masses."""
+
+    def __init__(self, device=0):
+        self._L = _lib.lib()
+        h = ctypes.c_void_p()
+        st = self._L.ma_create(int(device), ctypes.byref(h))
+        if st != 0:
+            raise EngineError(st, "ma_create failed (no GPU visible?)")
+        self._h = h
+        self.n = 0
+        self.mesh = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ma_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, st):
+        if st != 0:
+            msg = self._L.ma_last_error(self._h)
+            raise EngineError(st, msg.decode() if msg else "")
+
+    # ---- inputs -----------------------------------------------------------
+    def set_mesh(self, mesh):
+        vxy = np.ascontiguousarray(mesh.vxy, np.float64)
+        vval = np.ascontiguousarray(mesh.vval, np.float64)
+        tri = np.ascontiguousarray(mesh.tri, np.int32)
+        self._check(self._L.ma_set_mesh(self._h, vxy.shape[0], _ptr(vxy), _ptr(vval),
+                                        tri.shape[0], _ptr(tri)))
+        self.mesh = mesh
+        return self
+
+    def set_diracs(self, X, layout="xy"):
+        X = np.asarray(X, np.float64)
+        if layout == "colmajor":
+            Xc = np.ascontiguousarray(X.T).ravel()
+            self._check(self._L.ma_set_diracs(self._h, X.shape[0], _ptr(Xc), 1))
+        else:
+            X = np.ascontiguousarray(X.reshape(-1, 2))
+            self._check(self._L.ma_set_diracs(self._h, X.shape[0], _ptr(X), 0))
+        self.n = X.shape[0]
+        return self
+
+    # ---- Kantorovich ------------------------------------------------------
+    def kantorovich(self, w, precision="fp64", hessian=True):
+        w = np.ascontiguousarray(w, np.float64)
+        if w.shape != (self.n,):
+            raise ValueError("weights must have one entry per Dirac")
+        fval = ctypes.c_double()
+        nnz = ctypes.c_int64()
+        g = np.empty(self.n)
+        self._check(self._L.ma_kantorovich(self._h, _ptr(w), PRECISION[precision],
+                                           ctypes.byref(fval), _ptr(g), ctypes.byref(nnz)))
+        H = self.hessian() if hessian else None
+        return fval.value, g, H
+
+    def kantorovich_device(self, w_dev_ptr, precision="fp64", g_dev_ptr=None):
+        """Evaluation with device-resident weights (and masses): pointers are
+        device addresses (e.g. ``torch.Tensor.data_ptr()`` of a float64 tensor)."""
+        fval = ctypes.c_double()
+        nnz = ctypes.c_int64()
+        self._check(self._L.ma_kantorovich_device(
+            self._h, ctypes.c_void_p(w_dev_ptr), PRECISION[precision], ctypes.byref(fval),
+            ctypes.c_void_p(g_dev_ptr) if g_dev_ptr else None, ctypes.byref(nnz)))
+        return fval.value, nnz.value
+
+    def hessian(self):
+        info = self.eval_info()
+        nnz = int(info["nnz"])
+        ptr = np.empty(self.n + 1, np.int32)
+        col = np.empty(max(nnz, 1), np.int32)
+        val = np.empty(max(nnz, 1))
+        self._check(self._L.ma_hessian_copy(self._h, _ptr(ptr), _ptr(col), _ptr(val)))
+        return sp.csr_matrix((val[:nnz], col[:nnz], ptr), shape=(self.n, self.n))
+
+    def eval_info(self):
+        info = _lib.EvalInfo()
+        self._check(self._L.ma_get_eval_info(self._h, ctypes.byref(info)))
+        return {k: getattr(info, k) for k, _ in _lib.EvalInfo._fields_}
+
+    # ---- linear solve / Newton -------------------------------------------
+    def solve_laplacian(self, g, rel_tol=1e-12, maxiter=0):
+        g = np.ascontiguousarray(g, np.float64)
+        d = np.empty(self.n)
+        res = ctypes.c_double()
+        it = ctypes.c_int64()
+        self._check(self._L.ma_solve_laplacian(self._h, _ptr(g), _ptr(d), rel_tol, int(maxiter),
+                                               ctypes.byref(res), ctypes.byref(it)))
+        return d, res.value, it.value
+
+    def ot_solve(self, nu, x=None, eps_g=1e-7, maxiter=100, verbose=False, stop="l2",
+                 tol=1e-8, precision="fp64", cg_tol=0.0, cg_maxiter=20000,
+                 alpha_min=2.0 ** -30, raise_on_error=True):
+        nu = np.ascontiguousarray(nu, np.float64)
+        init = x is not None and len(x) == self.n
+        w = np.ascontiguousarray(x, np.float64).copy() if init else np.zeros(self.n)
+        o = _lib.SolveOpts()
+        self._L.ma_default_solve_opts(ctypes.byref(o))
+        o.precision = PRECISION[precision]
+        o.stop_rule = 0 if stop == "l2" else 1
+        o.eps_g = eps_g
+        o.tol = tol
+        o.maxiter = int(maxiter)
+        o.alpha_min = alpha_min
+        o.cg_tol = cg_tol
+        o.cg_maxiter = int(cg_maxiter)
+        o.verbose = int(bool(verbose))
+        s = _lib.Stats()
+        st = self._L.ma_ot_solve(self._h, _ptr(nu), _ptr(w), int(init), ctypes.byref(o),
+                                 ctypes.byref(s))
+        stats = Statistics(s.niter, s.neval, s.cg_iter, s.max_rel_err, s.l2_err, s.fval,
+                           s.ms_total, s.ms_eval, s.ms_linear, _lib.STATUS.get(st, str(st)),
+                           s.bad_index)
+        if st != 0 and raise_on_error and st not in (_lib.MA_NOT_CONVERGED,):
+            self._check(st)
+        return w, stats
+
+    # ---- moments / Lloyd --------------------------------------------------
+    def first_moment(self, w):
+        w = np.ascontiguousarray(w, np.float64)
+        m = np.empty(self.n)
+        mom = np.empty((self.n, 2))
+        self._check(self._L.ma_first_moment(self._h, _ptr(w), _ptr(m), _ptr(mom)))
+        return m, mom
+
+    def lloyd(self, w):
+        m, mom = self.first_moment(w)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return m, mom / m[:, None]
+
+    def adjoint(self, w):
+        w = np.ascontiguousarray(w, np.float64)
+        dY = np.empty((self.n, 2))
+        dval = np.empty(self.mesh.vxy.shape[0])
+        self._check(self._L.ma_adjoint(self._h, _ptr(w), _ptr(dY), _ptr(dval)))
+        return dY, dval
+
+
+# ---- reference-shaped free functions ---------------------------------------
+_DEFAULT = {}
+
+
+def _engine(mesh, X):
+    e = _DEFAULT.get("e")
+    if e is None:
+        e = Engine()
+        _DEFAULT["e"] = e
+    if _DEFAULT.get("mesh") is not mesh:
+        e.set_mesh(mesh)
+        _DEFAULT["mesh"] = mesh
+        _DEFAULT["X"] = None
+    if _DEFAULT.get("X") is not X:
+        e.set_diracs(X)
+        _DEFAULT["X"] = X
+    return e
+
+
+def kantorovich(mesh, X, weights, precision="fp64"):
+    """``MA::kantorovich``: returns (value, g = cell masses, h = sparse Hessian)."""
+    return _engine(mesh, X).kantorovich(weights, precision)
+
+
+def ot_solve(mesh, X, masses, x=None, eps_g=1e-7, maxiter=100, verbose=False, **kw):
+    """``MA::ot_solve``: returns (weights, Statistics)."""
+    return _engine(mesh, X).ot_solve(masses, x, eps_g=eps_g, maxiter=maxiter, verbose=verbose, **kw)
+
+
+def first_moment(mesh, X, weights):
+    return _engine(mesh, X).first_moment(weights)
+
+
+def lloyd(mesh, X, weights):
+    return _engine(mesh, X).lloyd(weights)

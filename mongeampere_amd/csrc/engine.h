@@ -1,0 +1,157 @@
+// Internal declarations shared by the engine's HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ma/ma.h"
+
+namespace ma {
+
+// ---------------------------------------------------------------------------
+// Device buffer (owned, grows on demand, never shrinks until destroyed).
+// ---------------------------------------------------------------------------
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t count) {
+    if (count <= n) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      p = nullptr;
+      n = 0;
+      if (e != hipSuccess) return e;
+    }
+    size_t c = count > 0 ? count : 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), c * sizeof(T));
+    if (e == hipSuccess) n = c;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Uniform square grid of 2^L x 2^L bins over a square region; bins are indexed
+// by the Morton (Z-order) code of their integer coordinates.
+struct Grid {
+  double x0, y0;  // lower-left corner
+  double h;       // bin side
+  int L;          // levels: G = 2^L bins per side
+  int G;
+  int64_t nbins;  // G * G
+};
+
+// Per-triangle data, counter-clockwise vertices, density a x + b y + c.
+struct TriData {
+  double x[3], y[3];
+  double a, b, c;
+};
+
+// Capacity of the fixed-size per-thread polygon buffers.
+constexpr int kMaxPoly = 32;
+
+struct EvalBuffers {
+  DBuf<double> w_s;          // weights in Morton order
+  DBuf<double> pyr;          // max-weight pyramid (all levels)
+  DBuf<int32_t> cand_cnt;    // per bin
+  DBuf<int32_t> cand;        // per bin x cap_cand
+  DBuf<int32_t> owner;       // per bin, best competitor of the last evaluation
+  DBuf<int32_t> rect;        // per Dirac: bin rectangle (x0, y0, x1, y1)
+  DBuf<double> mass_s, intg_s, fval_part;
+  DBuf<double> mom_s;        // first moments (2 per Dirac)
+  DBuf<int32_t> row_cnt;     // per Dirac, entries incl. diagonal
+  DBuf<int32_t> row_col;     // n x cap_row
+  DBuf<double> row_val;      // n x cap_row
+  DBuf<int32_t> csr_ptr, csr_col;   // Morton-order CSR
+  DBuf<double> csr_val;
+  DBuf<float> csr_val32;
+  DBuf<double> diag;
+  DBuf<int32_t> flags;       // [0] overflow poly, [1] overflow cand, [2] overflow row
+  DBuf<unsigned long long> counters;  // [0] fragments, [1] candidates, [2] max poly, [3] empty
+  DBuf<int32_t> scan_tmp;
+  DBuf<double> red_tmp;
+};
+
+struct SolverBuffers {
+  DBuf<double> w, m, nu, g, d, x, r, z, p, Ap, wtrial, red;
+  DBuf<float> r32, z32, p32, Ap32, x32;
+};
+
+}  // namespace ma
+
+struct ma_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[8] = {};
+  std::string err;
+
+  // mesh
+  int64_t nv = 0, nt = 0;
+  double dom[4] = {0, 0, 0, 0};  // domain bounding box
+  ma::DBuf<ma::TriData> tri;
+  ma::DBuf<int32_t> triv;  // ccw vertex ids (nt x 3)
+  ma::DBuf<double> vxy, vval;
+  bool have_mesh = false;
+
+  // Diracs and grid
+  int64_t n = 0;
+  bool have_diracs = false;
+  ma::Grid grid{};
+  ma::DBuf<double> pos;        // Morton-ordered positions (2 n)
+  ma::DBuf<int32_t> perm;      // Morton slot -> caller index
+  ma::DBuf<int32_t> iperm;     // caller index -> Morton slot
+  ma::DBuf<int32_t> bin_start; // Diracs of bin b: [bin_start[b], bin_start[b+1])
+  ma::DBuf<int32_t> tbin_start, tbin_items;
+  bool grid_valid = false;     // triangle bins built for the current grid
+
+  int cap_cand = 48;
+  int cap_row = ma::kMaxPoly + 1;
+
+  ma::EvalBuffers eb;
+  ma::SolverBuffers sb;
+  ma_eval_info info{};
+  bool have_eval = false;
+  int64_t nnz = 0;
+
+  // host scratch
+  std::vector<int64_t> pyr_off;
+};
+
+namespace ma {
+
+// Geometry (geometry.hip)
+hipError_t build_dirac_grid(ma_engine *e, const double *d_X, int layout);
+hipError_t build_triangle_bins(ma_engine *e);
+// Evaluates masses / value / Hessian (CSR in Morton order) at Morton-ordered
+// weights e->eb.w_s.  mode 0: Kantorovich, mode 1: first moments.
+ma_status evaluate(ma_engine *e, int mode, double *fval_out);
+// Copy helpers between caller order and Morton order.
+hipError_t to_morton(ma_engine *e, const double *src_caller, double *dst_morton, bool src_on_device);
+hipError_t from_morton(ma_engine *e, const double *src_morton, double *dst_caller, bool dst_on_device);
+hipError_t exclusive_scan(ma_engine *e, const int32_t *in, int32_t *out, int64_t n);
+hipError_t csr_to_caller(ma_engine *e, int32_t *rowptr, int32_t *col, double *val);
+ma_status adjoint(ma_engine *e, double *dY_caller, double *dval);
+
+}  // namespace ma
+
+#define MA_HIP_TRY(expr)                                                   \
+  do {                                                                     \
+    hipError_t _e = (expr);                                                \
+    if (_e != hipSuccess) {                                                \
+      e->err = std::string(#expr) + ": " + hipGetErrorString(_e);          \
+      return _e == hipErrorOutOfMemory ? MA_OUT_OF_MEMORY : MA_HIP_ERROR;  \
+    }                                                                      \
+  } while (0)
+
+#define MA_HIP_RET(expr)               \
+  do {                                 \
+    hipError_t _e = (expr);            \
+    if (_e != hipSuccess) return _e;   \
+  } while (0)

@@ -1,0 +1,766 @@
+// Engine object, C ABI entry points and the device-resident damped Newton
+// solver (include/MA/optimal_transport.hpp:41-193 behaviour) with a
+// Jacobi-preconditioned conjugate gradient on the gauge-fixed Laplacian.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+#include "engine.h"
+
+namespace ma {
+
+hipError_t device_sum(ma_engine *e, const double *x, int64_t n, double *d_out);
+
+constexpr int kVB = 256;      // threads per block for vector kernels
+constexpr int kVGrid = 512;   // fixed grid => fixed number of partial sums
+
+// ---------------------------------------------------------------------------
+// Mesh preparation: ccw orientation and per-face linear density a x + b y + c
+// (the plane through the three vertex values; include/MA/functions.hpp:55-80).
+// ---------------------------------------------------------------------------
+__global__ void k_tri_prep(const double *vxy, const double *vval, const int32_t *tri, int64_t nt,
+                           TriData *out, int32_t *triv, int32_t *bad) {
+  int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f >= nt) return;
+  int32_t id[3] = {tri[3 * f], tri[3 * f + 1], tri[3 * f + 2]};
+  double x[3], y[3], v[3];
+  for (int k = 0; k < 3; k++) {
+    x[k] = vxy[2 * id[k]];
+    y[k] = vxy[2 * id[k] + 1];
+    v[k] = vval[id[k]];
+  }
+  double D = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0]);
+  if (D < 0) {
+    double t;
+    t = x[1]; x[1] = x[2]; x[2] = t;
+    t = y[1]; y[1] = y[2]; y[2] = t;
+    t = v[1]; v[1] = v[2]; v[2] = t;
+    int32_t s = id[1]; id[1] = id[2]; id[2] = s;
+    D = -D;
+  }
+  if (!(D > 0)) atomicOr(bad, 1);
+  TriData T;
+  for (int k = 0; k < 3; k++) {
+    T.x[k] = x[k];
+    T.y[k] = y[k];
+  }
+  const double f1 = v[1] - v[0], f2 = v[2] - v[0];
+  const double ex1 = x[1] - x[0], ey1 = y[1] - y[0], ex2 = x[2] - x[0], ey2 = y[2] - y[0];
+  T.a = (f1 * ey2 - f2 * ey1) / D;
+  T.b = (ex1 * f2 - ex2 * f1) / D;
+  T.c = v[0] - T.a * x[0] - T.b * y[0];
+  out[f] = T;
+  for (int k = 0; k < 3; k++) triv[3 * f + k] = id[k];
+}
+
+// ---------------------------------------------------------------------------
+// Vector kernels (grid-stride, fixed grid, deterministic partial sums).
+// ---------------------------------------------------------------------------
+__device__ inline double block_sum(double v, double *s) {
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = kVB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  double r = s[0];
+  __syncthreads();
+  return r;
+}
+// sum of kVGrid partials, computed identically by every block
+__device__ inline double reduce_partials(const double *part, double *s) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < kVGrid; i += kVB) v += part[i];
+  return block_sum(v, s);
+}
+
+// y = A x on the gauge-fixed system (row/column gi removed; y[gi] = 0),
+// partial sums of x.y.
+__global__ void __launch_bounds__(kVB) k_spmv_dot(int64_t n, const int32_t *ptr,
+                                                  const int32_t *col, const double *val,
+                                                  const double *x, double *y, int32_t gi,
+                                                  double *part) {
+  __shared__ double s[kVB];
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
+    double v = 0.0;
+    if (i != gi) {
+      for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) {
+        const int32_t j = col[q];
+        if (j != gi) v += val[q] * x[j];
+      }
+    }
+    y[i] = v;
+    acc += x[i] * v;
+  }
+  acc = block_sum(acc, s);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// scal layout: [0..1] rz (ping-pong), [2] rr, [3] pAp, [4] bb
+__global__ void __launch_bounds__(kVB) k_cg_init(int64_t n, const double *b, const double *diag,
+                                                 int32_t gi, double *x, double *r, double *z,
+                                                 double *p, double *part_rz, double *part_rr) {
+  __shared__ double s[kVB];
+  double rz = 0.0, rr = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
+    const double ri = (i == gi) ? 0.0 : b[i];
+    const double di = diag[i];
+    const double zi = (di > 0.0) ? ri / di : 0.0;
+    x[i] = 0.0;
+    r[i] = ri;
+    z[i] = zi;
+    p[i] = zi;
+    rz += ri * zi;
+    rr += ri * ri;
+  }
+  rz = block_sum(rz, s);
+  rr = block_sum(rr, s);
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = rz;
+    part_rr[blockIdx.x] = rr;
+  }
+}
+__global__ void __launch_bounds__(kVB) k_cg_scal_init(const double *part_rz, const double *part_rr,
+                                                      double *scal) {
+  __shared__ double s[kVB];
+  double rz = reduce_partials(part_rz, s);
+  double rr = reduce_partials(part_rr, s);
+  if (threadIdx.x == 0) {
+    scal[0] = rz;
+    scal[2] = rr;
+    scal[4] = rr;
+    scal[5] = 0.0;  // done flag
+  }
+}
+// x += alpha p, r -= alpha Ap, z = r / diag; partials of r.z and r.r
+__global__ void __launch_bounds__(kVB) k_cg_update(int64_t n, int parity, const double *part_pap,
+                                                   double *scal, const double *diag, int32_t gi,
+                                                   double *x, double *r, double *z,
+                                                   const double *p, const double *Ap,
+                                                   double *part_rz, double *part_rr) {
+  __shared__ double s[kVB];
+  if (scal[5] != 0.0) return;
+  const double pap = reduce_partials(part_pap, s);
+  const double rz = scal[parity];
+  const double alpha = (pap > 0.0) ? rz / pap : 0.0;
+  double a = 0.0, c = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
+    if (i == gi) continue;
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    const double di = diag[i];
+    const double zi = (di > 0.0) ? ri / di : 0.0;
+    z[i] = zi;
+    a += ri * zi;
+    c += ri * ri;
+  }
+  a = block_sum(a, s);
+  c = block_sum(c, s);
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = a;
+    part_rr[blockIdx.x] = c;
+  }
+}
+// p = z + beta p; writes the new r.z into the other parity slot
+__global__ void __launch_bounds__(kVB) k_cg_dir(int64_t n, int parity, const double *part_rz,
+                                                const double *part_rr, double *scal, double tol2,
+                                                const double *z, double *p) {
+  __shared__ double s[kVB];
+  if (scal[5] != 0.0) return;
+  const double rz_new = reduce_partials(part_rz, s);
+  const double rr = reduce_partials(part_rr, s);
+  const double rz = scal[parity];
+  const double beta = (rz > 0.0) ? rz_new / rz : 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB)
+    p[i] = z[i] + beta * p[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[1 - parity] = rz_new;
+    scal[2] = rr;
+    if (rr <= tol2 * scal[4]) scal[5] = 1.0;
+  }
+}
+// Stop the remaining launches of a batch once converged.
+__global__ void k_noop() {}
+
+// Reductions for the Newton driver: min m, ||m - nu||^2, max |m-nu|/nu, nu.w,
+// argmin m.  g = m - nu is written.
+__global__ void __launch_bounds__(kVB) k_newton_stats(int64_t n, const double *m,
+                                                      const double *nu, const double *w,
+                                                      double *g, double *part) {
+  __shared__ double s[kVB];
+  double mn = HUGE_VAL, ss = 0.0, mr = 0.0, dot = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
+    const double gi = m[i] - nu[i];
+    g[i] = gi;
+    mn = fmin(mn, m[i]);
+    ss += gi * gi;
+    mr = fmax(mr, fabs(gi) / nu[i]);
+    dot += nu[i] * w[i];
+  }
+  // min / max via the same shared buffer
+  s[threadIdx.x] = mn;
+  __syncthreads();
+  for (int o = kVB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + o]);
+    __syncthreads();
+  }
+  mn = s[0];
+  __syncthreads();
+  s[threadIdx.x] = mr;
+  __syncthreads();
+  for (int o = kVB / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o) s[threadIdx.x] = fmax(s[threadIdx.x], s[threadIdx.x + o]);
+    __syncthreads();
+  }
+  mr = s[0];
+  __syncthreads();
+  ss = block_sum(ss, s);
+  dot = block_sum(dot, s);
+  if (threadIdx.x == 0) {
+    part[4 * blockIdx.x + 0] = mn;
+    part[4 * blockIdx.x + 1] = ss;
+    part[4 * blockIdx.x + 2] = mr;
+    part[4 * blockIdx.x + 3] = dot;
+  }
+}
+__global__ void k_axpy_to(int64_t n, const double *x, double a, const double *d, double *y) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] + a * d[i];
+}
+__global__ void k_neg(int64_t n, double *x) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = -x[i];
+}
+
+__global__ void k_split2(const double *m2, int64_t n, int c, double *o) {
+  int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k < n) o[k] = m2[2 * k + c];
+}
+
+struct NewtonStats {
+  double minm, l2, maxrel, nuw;
+};
+
+static hipError_t newton_stats(ma_engine *e, const double *m, const double *nu, const double *w,
+                               double *g, NewtonStats &st) {
+  SolverBuffers &S = e->sb;
+  MA_HIP_RET(S.red.ensure(4 * kVGrid));
+  hipLaunchKernelGGL(k_newton_stats, dim3(kVGrid), dim3(kVB), 0, e->stream, e->n, m, nu, w, g,
+                     S.red.p);
+  std::vector<double> h(4 * kVGrid);
+  MA_HIP_RET(hipMemcpyAsync(h.data(), S.red.p, sizeof(double) * 4 * kVGrid,
+                            hipMemcpyDeviceToHost, e->stream));
+  MA_HIP_RET(hipStreamSynchronize(e->stream));
+  st.minm = HUGE_VAL;
+  st.maxrel = 0.0;
+  double ss = 0.0, dot = 0.0;
+  for (int b = 0; b < kVGrid; b++) {
+    st.minm = std::min(st.minm, h[4 * b]);
+    ss += h[4 * b + 1];
+    st.maxrel = std::max(st.maxrel, h[4 * b + 2]);
+    dot += h[4 * b + 3];
+  }
+  st.l2 = std::sqrt(ss);
+  st.nuw = dot;
+  return hipSuccess;
+}
+
+// Solve A x = b on the gauge-fixed Hessian (Morton order) with Jacobi PCG.
+static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, int maxiter,
+                      int64_t *iters, double *rel_res) {
+  SolverBuffers &S = e->sb;
+  EvalBuffers &B = e->eb;
+  const int64_t n = e->n;
+  int32_t gi = 0;
+  {
+    // gauge: the caller's last Dirac (its weight is never moved,
+    // include/MA/optimal_transport.hpp:82-86)
+    MA_HIP_RET(hipMemcpyAsync(&gi, e->iperm.p + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
+                              e->stream));
+    MA_HIP_RET(hipStreamSynchronize(e->stream));
+  }
+  MA_HIP_RET(S.r.ensure((size_t)n));
+  MA_HIP_RET(S.z.ensure((size_t)n));
+  MA_HIP_RET(S.p.ensure((size_t)n));
+  MA_HIP_RET(S.Ap.ensure((size_t)n));
+  MA_HIP_RET(S.red.ensure(4 * kVGrid + 16));
+  double *part_a = S.red.p, *part_b = S.red.p + kVGrid, *part_c = S.red.p + 2 * kVGrid;
+  double *scal = S.red.p + 4 * kVGrid;
+  hipLaunchKernelGGL(k_cg_init, dim3(kVGrid), dim3(kVB), 0, e->stream, n, b, B.diag.p, gi, x,
+                     S.r.p, S.z.p, S.p.p, part_a, part_b);
+  hipLaunchKernelGGL(k_cg_scal_init, dim3(1), dim3(kVB), 0, e->stream, part_a, part_b, scal);
+  const double tol2 = rel_tol * rel_tol;
+  int it = 0;
+  double sc[6];
+  MA_HIP_RET(hipMemcpyAsync(sc, scal, sizeof sc, hipMemcpyDeviceToHost, e->stream));
+  MA_HIP_RET(hipStreamSynchronize(e->stream));
+  if (sc[4] == 0.0) {
+    *iters = 0;
+    *rel_res = 0.0;
+    return hipSuccess;
+  }
+  const int batch = 32;
+  while (it < maxiter) {
+    for (int q = 0; q < batch && it < maxiter; q++, it++) {
+      const int parity = it & 1;
+      hipLaunchKernelGGL(k_spmv_dot, dim3(kVGrid), dim3(kVB), 0, e->stream, n, B.csr_ptr.p,
+                         B.csr_col.p, B.csr_val.p, S.p.p, S.Ap.p, gi, part_c);
+      hipLaunchKernelGGL(k_cg_update, dim3(kVGrid), dim3(kVB), 0, e->stream, n, parity, part_c,
+                         scal, B.diag.p, gi, x, S.r.p, S.z.p, S.p.p, S.Ap.p, part_a, part_b);
+      hipLaunchKernelGGL(k_cg_dir, dim3(kVGrid), dim3(kVB), 0, e->stream, n, parity, part_a,
+                         part_b, scal, tol2, S.z.p, S.p.p);
+    }
+    MA_HIP_RET(hipGetLastError());
+    MA_HIP_RET(hipMemcpyAsync(sc, scal, sizeof sc, hipMemcpyDeviceToHost, e->stream));
+    MA_HIP_RET(hipStreamSynchronize(e->stream));
+    if (sc[5] != 0.0) break;
+  }
+  *iters = it;
+  *rel_res = std::sqrt(sc[2] / sc[4]);
+  return hipSuccess;
+}
+
+}  // namespace ma
+
+using namespace ma;
+
+static ma_status need(ma_engine *e) {
+  if (!e) return MA_BAD_INPUT;
+  if (!e->have_mesh || !e->have_diracs) {
+    e->err = "mesh and Diracs must be set first";
+    return MA_NO_DATA;
+  }
+  if (!e->grid_valid) {
+    hipError_t r = build_triangle_bins(e);
+    if (r != hipSuccess) {
+      e->err = std::string("triangle binning: ") + hipGetErrorString(r);
+      return MA_HIP_ERROR;
+    }
+    e->grid_valid = true;
+  }
+  return MA_OK;
+}
+
+static void choose_grid(ma_engine *e, const double *X, int64_t n, int layout) {
+  double x0 = e->dom[0], y0 = e->dom[1], x1 = e->dom[2], y1 = e->dom[3];
+  if (!e->have_mesh) {
+    x0 = y0 = HUGE_VAL;
+    x1 = y1 = -HUGE_VAL;
+  }
+  for (int64_t i = 0; i < n; i++) {
+    double x = layout == MA_XY_COLMAJOR ? X[i] : X[2 * i];
+    double y = layout == MA_XY_COLMAJOR ? X[n + i] : X[2 * i + 1];
+    x0 = std::min(x0, x);
+    x1 = std::max(x1, x);
+    y0 = std::min(y0, y);
+    y1 = std::max(y1, y);
+  }
+  double side = std::max(x1 - x0, y1 - y0);
+  if (!(side > 0)) side = 1.0;
+  side *= 1.0 + 1e-6;
+  const double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+  int L = (int)std::lround(std::log(std::max(1.0, (double)n / 1.5)) / std::log(4.0));
+  L = std::max(0, std::min(L, 12));
+  Grid g;
+  g.L = L;
+  g.G = 1 << L;
+  g.nbins = (int64_t)g.G * g.G;
+  g.h = side / g.G;
+  g.x0 = cx - 0.5 * side;
+  g.y0 = cy - 0.5 * side;
+  e->grid = g;
+}
+
+extern "C" {
+
+void ma_default_solve_opts(ma_solve_opts *o) {
+  if (!o) return;
+  o->precision = MA_FP64;
+  o->stop_rule = MA_STOP_L2;
+  o->eps_g = 1e-7;
+  o->tol = 1e-8;
+  o->maxiter = 100;
+  o->alpha_min = std::ldexp(1.0, -30);
+  o->cg_tol = 0.0;
+  o->cg_maxiter = 20000;
+  o->verbose = 0;
+}
+
+ma_status ma_create(int device, ma_engine **out) {
+  if (!out) return MA_BAD_INPUT;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0)
+    return MA_HIP_ERROR;
+  if (hipSetDevice(device) != hipSuccess) return MA_HIP_ERROR;
+  ma_engine *e = new ma_engine();
+  e->device = device;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return MA_HIP_ERROR;
+  }
+  for (auto &ev : e->ev) hipEventCreate(&ev);
+  *out = e;
+  return MA_OK;
+}
+
+void ma_destroy(ma_engine *e) {
+  if (!e) return;
+  hipSetDevice(e->device);
+  hipStreamSynchronize(e->stream);
+  for (auto &ev : e->ev) hipEventDestroy(ev);
+  hipStreamDestroy(e->stream);
+  e->tri.release();
+  e->triv.release();
+  e->vxy.release();
+  e->vval.release();
+  e->pos.release();
+  e->perm.release();
+  e->iperm.release();
+  e->bin_start.release();
+  e->tbin_start.release();
+  e->tbin_items.release();
+  EvalBuffers &B = e->eb;
+  B.w_s.release(); B.pyr.release(); B.cand_cnt.release(); B.cand.release(); B.owner.release();
+  B.rect.release(); B.mass_s.release(); B.intg_s.release(); B.fval_part.release();
+  B.mom_s.release(); B.row_cnt.release(); B.row_col.release(); B.row_val.release();
+  B.csr_ptr.release(); B.csr_col.release(); B.csr_val.release(); B.csr_val32.release();
+  B.diag.release(); B.flags.release(); B.counters.release(); B.scan_tmp.release();
+  B.red_tmp.release();
+  SolverBuffers &S = e->sb;
+  for (DBuf<double> *b : {&S.w, &S.m, &S.nu, &S.g, &S.d, &S.x, &S.r, &S.z, &S.p, &S.Ap,
+                          &S.wtrial, &S.red})
+    b->release();
+  for (DBuf<float> *b : {&S.r32, &S.z32, &S.p32, &S.Ap32, &S.x32}) b->release();
+  delete e;
+}
+
+const char *ma_last_error(const ma_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+ma_status ma_set_mesh(ma_engine *e, int64_t nv, const double *vxy, const double *vval,
+                      int64_t nt, const int32_t *tri) {
+  if (!e || nv < 3 || nt < 1 || !vxy || !vval || !tri) return MA_BAD_INPUT;
+  if (nt > (int64_t)INT32_MAX / 4 || nv > (int64_t)INT32_MAX) return MA_BAD_INPUT;
+  hipSetDevice(e->device);
+  for (int64_t f = 0; f < 3 * nt; f++)
+    if (tri[f] < 0 || tri[f] >= nv) {
+      e->err = "triangle vertex index out of range";
+      return MA_BAD_INPUT;
+    }
+  double x0 = HUGE_VAL, y0 = HUGE_VAL, x1 = -HUGE_VAL, y1 = -HUGE_VAL;
+  for (int64_t i = 0; i < nv; i++) {
+    if (!std::isfinite(vxy[2 * i]) || !std::isfinite(vxy[2 * i + 1]) || !std::isfinite(vval[i])) {
+      e->err = "non-finite mesh data";
+      return MA_BAD_INPUT;
+    }
+    x0 = std::min(x0, vxy[2 * i]);
+    x1 = std::max(x1, vxy[2 * i]);
+    y0 = std::min(y0, vxy[2 * i + 1]);
+    y1 = std::max(y1, vxy[2 * i + 1]);
+  }
+  // a mesh vertex outside the current grid invalidates the Dirac binning
+  if (e->have_diracs && (x0 < e->grid.x0 || y0 < e->grid.y0 ||
+                         x1 > e->grid.x0 + e->grid.G * e->grid.h ||
+                         y1 > e->grid.y0 + e->grid.G * e->grid.h)) {
+    e->have_diracs = false;
+  }
+  e->dom[0] = x0; e->dom[1] = y0; e->dom[2] = x1; e->dom[3] = y1;
+  MA_HIP_TRY(e->vxy.ensure((size_t)(2 * nv)));
+  MA_HIP_TRY(e->vval.ensure((size_t)nv));
+  MA_HIP_TRY(e->tri.ensure((size_t)nt));
+  MA_HIP_TRY(e->triv.ensure((size_t)(3 * nt)));
+  DBuf<int32_t> dtri, bad;
+  MA_HIP_TRY(dtri.ensure((size_t)(3 * nt)));
+  MA_HIP_TRY(bad.ensure(1));
+  MA_HIP_TRY(hipMemcpyAsync(e->vxy.p, vxy, sizeof(double) * 2 * nv, hipMemcpyHostToDevice, e->stream));
+  MA_HIP_TRY(hipMemcpyAsync(e->vval.p, vval, sizeof(double) * nv, hipMemcpyHostToDevice, e->stream));
+  MA_HIP_TRY(hipMemcpyAsync(dtri.p, tri, sizeof(int32_t) * 3 * nt, hipMemcpyHostToDevice, e->stream));
+  MA_HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int32_t), e->stream));
+  hipLaunchKernelGGL(k_tri_prep, dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, e->stream,
+                     e->vxy.p, e->vval.p, dtri.p, nt, e->tri.p, e->triv.p, bad.p);
+  int32_t hb = 0;
+  MA_HIP_TRY(hipMemcpyAsync(&hb, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  dtri.release();
+  bad.release();
+  if (hb) {
+    e->err = "degenerate (zero-area) triangle";
+    return MA_BAD_INPUT;
+  }
+  e->nv = nv;
+  e->nt = nt;
+  e->have_mesh = true;
+  e->grid_valid = false;
+  e->have_eval = false;
+  return MA_OK;
+}
+
+ma_status ma_set_diracs(ma_engine *e, int64_t n, const double *X, int layout) {
+  if (!e || n < 1 || !X || (layout != MA_XY_INTERLEAVED && layout != MA_XY_COLMAJOR))
+    return MA_BAD_INPUT;
+  if (n > (int64_t)INT32_MAX / 8) return MA_BAD_INPUT;
+  hipSetDevice(e->device);
+  for (int64_t i = 0; i < 2 * n; i++)
+    if (!std::isfinite(X[i])) {
+      e->err = "non-finite Dirac position";
+      return MA_BAD_INPUT;
+    }
+  e->n = n;
+  choose_grid(e, X, n, layout);
+  DBuf<double> dX;
+  MA_HIP_TRY(dX.ensure((size_t)(2 * n)));
+  MA_HIP_TRY(hipMemcpyAsync(dX.p, X, sizeof(double) * 2 * n, hipMemcpyHostToDevice, e->stream));
+  MA_HIP_TRY(build_dirac_grid(e, dX.p, layout));
+  dX.release();
+  e->have_diracs = true;
+  e->grid_valid = false;
+  e->have_eval = false;
+  return MA_OK;
+}
+
+static ma_status eval_morton(ma_engine *e, int precision, double *fval) {
+  (void)precision;
+  return evaluate(e, 0, fval);
+}
+
+ma_status ma_kantorovich(ma_engine *e, const double *w, int precision, double *fval, double *g,
+                         int64_t *nnz) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!w) return MA_BAD_INPUT;
+  MA_HIP_TRY(e->eb.w_s.ensure((size_t)e->n));
+  MA_HIP_TRY(to_morton(e, w, e->eb.w_s.p, false));
+  double fv = 0.0;
+  s = eval_morton(e, precision, &fv);
+  if (s != MA_OK) return s;
+  if (fval) *fval = fv;
+  if (g) MA_HIP_TRY(from_morton(e, e->eb.mass_s.p, g, false));
+  if (nnz) *nnz = e->nnz;
+  return MA_OK;
+}
+
+ma_status ma_kantorovich_device(ma_engine *e, const double *d_w, int precision, double *fval,
+                                double *d_g, int64_t *nnz) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!d_w) return MA_BAD_INPUT;
+  MA_HIP_TRY(e->eb.w_s.ensure((size_t)e->n));
+  MA_HIP_TRY(to_morton(e, d_w, e->eb.w_s.p, true));
+  double fv = 0.0;
+  s = eval_morton(e, precision, &fv);
+  if (s != MA_OK) return s;
+  if (fval) *fval = fv;
+  if (d_g) {
+    MA_HIP_TRY(from_morton(e, e->eb.mass_s.p, d_g, true));
+    MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  }
+  if (nnz) *nnz = e->nnz;
+  return MA_OK;
+}
+
+ma_status ma_hessian_copy(ma_engine *e, int32_t *rowptr, int32_t *col, double *val) {
+  if (!e || !rowptr || !col || !val) return MA_BAD_INPUT;
+  if (!e->have_eval) {
+    e->err = "no evaluation yet";
+    return MA_NO_DATA;
+  }
+  MA_HIP_TRY(csr_to_caller(e, rowptr, col, val));
+  return MA_OK;
+}
+
+ma_status ma_first_moment(ma_engine *e, const double *w, double *mass, double *mom) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!w) return MA_BAD_INPUT;
+  MA_HIP_TRY(e->eb.w_s.ensure((size_t)e->n));
+  MA_HIP_TRY(to_morton(e, w, e->eb.w_s.p, false));
+  s = evaluate(e, 1, nullptr);
+  if (s != MA_OK) return s;
+  if (mass) MA_HIP_TRY(from_morton(e, e->eb.mass_s.p, mass, false));
+  if (mom) {
+    // interleaved (x, y) per Dirac: scatter both components
+    std::vector<double> mx(e->n), my(e->n);
+    DBuf<double> tmp;
+    MA_HIP_TRY(tmp.ensure((size_t)e->n));
+    for (int c = 0; c < 2; c++) {
+      hipLaunchKernelGGL(k_split2, dim3((unsigned)((e->n + 127) / 128)), dim3(128), 0,
+                         e->stream, e->eb.mom_s.p, e->n, c, tmp.p);
+      MA_HIP_TRY(from_morton(e, tmp.p, c == 0 ? mx.data() : my.data(), false));
+    }
+    tmp.release();
+    for (int64_t i = 0; i < e->n; i++) {
+      mom[2 * i] = mx[i];
+      mom[2 * i + 1] = my[i];
+    }
+  }
+  return MA_OK;
+}
+
+ma_status ma_solve_laplacian(ma_engine *e, const double *g, double *d, double rel_tol,
+                             int maxiter, double *res, int64_t *iters) {
+  if (!e || !g || !d) return MA_BAD_INPUT;
+  if (!e->have_eval) {
+    e->err = "no evaluation yet";
+    return MA_NO_DATA;
+  }
+  SolverBuffers &S = e->sb;
+  MA_HIP_TRY(S.g.ensure((size_t)e->n));
+  MA_HIP_TRY(S.d.ensure((size_t)e->n));
+  MA_HIP_TRY(to_morton(e, g, S.g.p, false));
+  int64_t it = 0;
+  double rr = 0.0;
+  MA_HIP_TRY(pcg(e, S.g.p, S.d.p, rel_tol > 0 ? rel_tol : 1e-12,
+                 maxiter > 0 ? maxiter : 100000, &it, &rr));
+  MA_HIP_TRY(from_morton(e, S.d.p, d, false));
+  if (res) *res = rr;
+  if (iters) *iters = it;
+  return MA_OK;
+}
+
+ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
+                      const ma_solve_opts *opts, ma_stats *stats) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!nu || !w) return MA_BAD_INPUT;
+  ma_solve_opts o;
+  ma_default_solve_opts(&o);
+  if (opts) o = *opts;
+  const auto t_start = std::chrono::steady_clock::now();
+  const int64_t n = e->n;
+  SolverBuffers &S = e->sb;
+  EvalBuffers &B = e->eb;
+  for (DBuf<double> *b : {&S.w, &S.nu, &S.g, &S.d, &S.wtrial}) MA_HIP_TRY(b->ensure((size_t)n));
+  MA_HIP_TRY(B.w_s.ensure((size_t)n));
+  MA_HIP_TRY(to_morton(e, nu, S.nu.p, false));
+  if (init_w) {
+    MA_HIP_TRY(to_morton(e, w, S.w.p, false));
+  } else {
+    MA_HIP_TRY(hipMemsetAsync(S.w.p, 0, sizeof(double) * n, e->stream));
+  }
+  ma_stats st{};
+  double ms_eval = 0.0, ms_lin = 0.0;
+  auto eval_at = [&](const double *wm, double &fval, NewtonStats &ns) -> ma_status {
+    auto t0 = std::chrono::steady_clock::now();
+    hipMemcpyAsync(B.w_s.p, wm, sizeof(double) * n, hipMemcpyDeviceToDevice, e->stream);
+    ma_status r = eval_morton(e, o.precision, &fval);
+    st.neval++;
+    if (r != MA_OK) return r;
+    hipError_t he = newton_stats(e, B.mass_s.p, S.nu.p, wm, S.g.p, ns);
+    if (he != hipSuccess) {
+      e->err = hipGetErrorString(he);
+      return MA_HIP_ERROR;
+    }
+    ms_eval += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return MA_OK;
+  };
+  double fval = 0.0;
+  NewtonStats ns{};
+  s = eval_at(S.w.p, fval, ns);
+  if (s != MA_OK) return s;
+  double numin = 0.0;
+  {
+    std::vector<double> hnu(nu, nu + n);
+    numin = *std::min_element(hnu.begin(), hnu.end());
+  }
+  const double eps0 = std::min(ns.minm, numin) / 2.0;
+  ma_status result = MA_OK;
+  if (eps0 <= 0) {
+    // include/MA/optimal_transport.hpp:137-148 (empty initial cell)
+    std::vector<double> m(n);
+    from_morton(e, B.mass_s.p, m.data(), false);
+    st.bad_index = std::min_element(m.begin(), m.end()) - m.begin();
+    e->err = "Laguerre cell of Dirac " + std::to_string(st.bad_index) + " is empty";
+    result = MA_EMPTY_CELL;
+  }
+  auto keep_going = [&](const NewtonStats &q) {
+    return o.stop_rule == MA_STOP_L2 ? (q.l2 >= o.eps_g) : (q.maxrel > o.tol);
+  };
+  int64_t niter = 0;
+  const double l2_0 = ns.l2;
+  while (result == MA_OK && keep_going(ns)) {
+    if (niter++ > o.maxiter) {  // `niter++ <= maxiter` (optimal_transport.hpp:151)
+      result = MA_NOT_CONVERGED;
+      break;
+    }
+    // Newton direction d = -H^-1 g (gauge fixed)
+    auto t0 = std::chrono::steady_clock::now();
+    double rel = o.cg_tol;
+    if (rel <= 0) {
+      // inexact Newton forcing term, tightened as the residual shrinks
+      rel = std::min(1e-3, std::max(1e-12, 1e-2 * ns.l2 / std::max(l2_0, 1e-300)));
+    }
+    int64_t it = 0;
+    double rr = 0.0;
+    MA_HIP_TRY(pcg(e, S.g.p, S.d.p, rel, o.cg_maxiter, &it, &rr));
+    hipLaunchKernelGGL(k_neg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, n, S.d.p);
+    st.cg_iter += it;
+    MA_HIP_TRY(hipStreamSynchronize(e->stream));
+    ms_lin += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    // backtracking (optimal_transport.hpp:158-176)
+    double alpha = 1.0;
+    const double n0 = ns.l2;
+    const NewtonStats ns0 = ns;
+    while (true) {
+      hipLaunchKernelGGL(k_axpy_to, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, n,
+                         S.w.p, alpha, S.d.p, S.wtrial.p);
+      s = eval_at(S.wtrial.p, fval, ns);
+      if (s != MA_OK) return s;
+      if (ns.minm >= eps0 && ns.l2 <= (1.0 - alpha / 2.0) * n0) break;
+      alpha *= 0.5;
+      if (alpha < o.alpha_min) {
+        result = MA_LINESEARCH_FAIL;
+        e->err = "step length below alpha_min";
+        ns = ns0;
+        break;
+      }
+    }
+    if (result != MA_OK) break;
+    std::swap(S.w.p, S.wtrial.p);
+    if (o.verbose)
+      fprintf(stderr, "it %lld: f=%.17g |df|=%.6g maxrel=%.3g tau=%g eval=%lld cg=%lld\n",
+              (long long)niter, fval - ns.nuw, ns.l2, ns.maxrel, alpha, (long long)st.neval,
+              (long long)it);
+  }
+  st.niter = niter;
+  st.max_rel_err = ns.maxrel;
+  st.l2_err = ns.l2;
+  st.fval = fval - ns.nuw;
+  st.ms_eval = ms_eval;
+  st.ms_linear = ms_lin;
+  MA_HIP_TRY(from_morton(e, S.w.p, w, false));
+  st.ms_total =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (stats) *stats = st;
+  return result;
+}
+
+ma_status ma_adjoint(ma_engine *e, const double *w, double *dY, double *dval) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!w) return MA_BAD_INPUT;
+  MA_HIP_TRY(e->eb.w_s.ensure((size_t)e->n));
+  MA_HIP_TRY(to_morton(e, w, e->eb.w_s.p, false));
+  return adjoint(e, dY, dval);
+}
+
+ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info) {
+  if (!e || !info) return MA_BAD_INPUT;
+  *info = e->info;
+  return MA_OK;
+}
+
+}  // extern "C"
+
+namespace ma {
+ma_status adjoint(ma_engine *e, double *dY, double *dval) {
+  (void)dY;
+  (void)dval;
+  e->err = "adjoint not available";
+  return MA_NO_DATA;
+}
+}  // namespace ma

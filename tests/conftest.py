@@ -1,0 +1,49 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the C ABI on device 0)")
+    config.addinivalue_line("markers", "slow: large inputs")
+
+
+def load_golden(name):
+    from mongeampere_amd.configs import Mesh
+
+    with open(os.path.join(GOLDEN, f"{name}.json")) as f:
+        d = json.load(f)
+    mesh = Mesh(np.array(d["vxy"], np.float64), np.array(d["vval"], np.float64),
+                np.array(d["tri"], np.int32))
+    X = np.array(d["X"], np.float64)
+    w = np.array(d["w"], np.float64)
+    e = d["expected"]
+    n = len(w)
+    H = np.zeros((n, n))
+    pattern = set()
+    for i, j, v in e["hessian"]:
+        H[i, j] += v
+        pattern.add((i, j))
+    return dict(name=name, mesh=mesh, X=X, w=w, fval=e["fval"], mass=np.array(e["mass"]),
+                intg=np.array(e["intg"]), mom=np.array(e["mom"]), H=H, pattern=pattern)
+
+
+GOLDEN_CASES = sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json"))
+
+
+@pytest.fixture(scope="session")
+def engine():
+    from mongeampere_amd.api import Engine
+
+    e = Engine(0)
+    yield e
+    e.close()

@@ -1,0 +1,114 @@
+"""GPU parity of the Kantorovich evaluation (FP64 mode) through the C ABI.
+
+Checked against (a) the exact-rational golden vectors in tests/golden and
+(b) the FP64 CPU oracle on seeded inputs, to 1e-10 relative (FP64 mode).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN_CASES, load_golden
+from mongeampere_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10  # FP64 mode: relative to the largest entry of each quantity
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _pattern(H, rel=0.0):
+    A = H.toarray() if hasattr(H, "toarray") else H
+    thr = rel * np.abs(A).max()
+    return set(zip(*np.nonzero(np.abs(A) > thr)))
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_golden_exact(engine, name):
+    c = load_golden(name)
+    print(f"case {name}", flush=True)
+    engine.set_mesh(c["mesh"]).set_diracs(c["X"])
+    fval, g, H = engine.kantorovich(c["w"])
+    assert _rel(g, c["mass"]) <= 1e-12
+    assert abs(fval - c["fval"]) <= 1e-12 * max(abs(c["fval"]), 1e-300)
+    if c["H"].any():
+        assert _rel(H.toarray(), c["H"]) <= 1e-12
+    else:
+        assert H.nnz == 0
+    # adjacency identical except for entries at the level of exact ties
+    assert _pattern(H, 1e-12) == _pattern(c["H"], 1e-12)
+    m, mom = engine.first_moment(c["w"])
+    assert _rel(m, c["mass"]) <= 1e-12
+    assert np.abs(mom - c["mom"]).max() <= 1e-12 * max(np.abs(c["mom"]).max(), 1e-300)
+
+
+def random_problem(seed):
+    rng = np.random.Generator(np.random.PCG64(1000 + seed))
+    n = int(rng.integers(4, 40))
+    N = int(rng.integers(2, 400))
+    vxy, tri = configs.grid_mesh(n, 0.0, 0.0, float(rng.uniform(0.5, 2.0)), float(rng.uniform(0.5, 2.0)))
+    vval = rng.uniform(0.2, 2.0, vxy.shape[0])
+    mesh = configs.Mesh(vxy, vval, tri)
+    lo = vxy.min(axis=0) - 0.1
+    hi = vxy.max(axis=0) + 0.1
+    X = rng.uniform(lo, hi, size=(N, 2))
+    scale = float(((hi - lo) ** 2).sum()) / N
+    w = rng.uniform(-1, 1, N) * scale * float(rng.choice([0.0, 0.1, 1.0, 5.0]))
+    return mesh, X, w
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_random_vs_oracle(engine, seed):
+    mesh, X, w = random_problem(seed)
+    print(f"seed {seed} N={X.shape[0]} T={mesh.tri.shape[0]}", flush=True)
+    engine.set_mesh(mesh).set_diracs(X)
+    fval, g, H = engine.kantorovich(w)
+    of, og, oH = oracle.kantorovich(mesh, X, w)
+    assert _rel(g, og) <= TOL
+    assert abs(fval - of) <= TOL * max(abs(of), 1e-300)
+    if oH.nnz:
+        assert _rel(H.toarray(), oH.toarray()) <= TOL
+        # adjacency identical except for entries at the level of ties
+        assert _pattern(H, 1e-9) == _pattern(oH, 1e-9)
+
+
+@pytest.mark.parametrize("wscale", [0.0, 1.0])
+def test_config1_vs_oracle(engine, wscale):
+    p = configs.config1()
+    w = np.random.default_rng(5).normal(size=p.X.shape[0]) * 1e-4 * wscale
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    fval, g, H = engine.kantorovich(w)
+    of, og, oH = oracle.kantorovich(p.mesh, p.X, w)
+    assert _rel(g, og) <= TOL
+    assert abs(fval - of) <= TOL * abs(of)
+    assert _rel(H.toarray(), oH.toarray()) <= TOL
+    assert _pattern(H, 1e-9) == _pattern(oH, 1e-9)
+
+
+@pytest.mark.slow
+def test_config2_vs_oracle(engine):
+    p = configs.config2()
+    w = np.zeros(p.X.shape[0])
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    fval, g, H = engine.kantorovich(w)
+    of, og, oH = oracle.kantorovich(p.mesh, p.X, w)
+    assert _rel(g, og) <= TOL
+    assert abs(fval - of) <= TOL * abs(of)
+    D = (H - oH)
+    assert abs(D).max() <= TOL * abs(oH).max()
+
+
+def test_invariants_config3_scaled(engine):
+    """Size-independent properties at a large size: total mass, H 1 = 0,
+    symmetry, non-positive off-diagonals."""
+    p = configs.config3(N=200_000, n=512)
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    fval, g, H = engine.kantorovich(np.zeros(p.X.shape[0]))
+    assert abs(g.sum() - 1.0) <= 1e-12
+    assert np.abs(H @ np.ones(H.shape[0])).max() <= 1e-10 * abs(H.diagonal()).max()
+    assert abs(H - H.T).max() <= 1e-10 * abs(H.diagonal()).max()
+    off = H - __import__("scipy.sparse", fromlist=["diags"]).diags(H.diagonal())
+    assert off.max() <= 0.0
+    assert (g > 0).all()

@@ -1,0 +1,62 @@
+"""GPU Newton solver parity (FP64 mode) through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle
+from mongeampere_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+
+def test_solve_config1_maxrel(engine):
+    """Config 1: damped Newton from w=0 to max |m-nu|/nu <= 1e-9; masses at
+    the GPU solution checked with the oracle; weights equal the oracle's
+    Newton solution modulo the gauge (the last weight is never moved)."""
+    p = configs.config1()
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-9)
+    assert st.status == "MA_OK"
+    assert st.max_rel_err <= 1e-9
+    _, og, _ = oracle.kantorovich(p.mesh, p.X, w)
+    assert np.max(np.abs(og - p.nu) / p.nu) <= 2e-9
+    ow, ost = oracle.ot_solve(p.mesh, p.X, p.nu, stop="maxrel", tol=1e-9)
+    assert ost.status == "ok"
+    assert abs(w[-1]) == 0.0  # gauge: w_{N-1} stays at its initial value
+    assert np.max(np.abs(w - ow)) <= 1e-6 * np.max(np.abs(ow))
+
+
+def test_solve_reference_rule(engine):
+    """The reference stopping rule ||m - nu||_2 < eps_g."""
+    p = configs.config1(N=400, n=24)
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    w, st = engine.ot_solve(p.nu, eps_g=1e-7)
+    assert st.status == "MA_OK"
+    assert st.l2_err < 1e-7
+    _, og, _ = oracle.kantorovich(p.mesh, p.X, w)
+    assert np.linalg.norm(og - p.nu) < 1.5e-7
+
+
+def test_solve_laplacian_vs_direct(engine):
+    """Gauge-fixed linear solve vs the oracle's sparse direct solve
+    (include/MA/optimal_transport.hpp:41-87)."""
+    p = configs.config1()
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    _, g, H = engine.kantorovich(np.zeros(p.X.shape[0]))
+    rhs = g - p.nu
+    d, res, it = engine.solve_laplacian(rhs, rel_tol=1e-13)
+    od, err = oracle.solve_laplacian_matrix(H, rhs)
+    assert d[-1] == 0.0
+    assert np.max(np.abs(d - od)) <= 1e-8 * np.max(np.abs(od))
+
+
+def test_empty_cell_status(engine):
+    """A Dirac whose cell is empty at w=0 makes the solve stop with
+    MA_EMPTY_CELL and report its index (reference: eps0 <= 0 early return,
+    include/MA/optimal_transport.hpp:137-148)."""
+    p = configs.config1(N=50, n=9)
+    X = p.X.copy()
+    X[7] = X[3]  # coincident Diracs: one of them is hidden
+    engine.set_mesh(p.mesh).set_diracs(X)
+    w, st = engine.ot_solve(p.nu, raise_on_error=False)
+    assert st.status == "MA_EMPTY_CELL"
+    assert st.bad_index in (3, 7)
