@@ -77,11 +77,16 @@ typedef struct {
  * the last evaluation. */
 typedef struct {
   double ms_total, ms_weights, ms_candidates, ms_cells, ms_assemble;
+  double ms_cell_build, ms_pairs, ms_frag; /* parts of ms_cells             */
   int64_t nnz;          /* Hessian nonzeros (incl. diagonal)                  */
   int64_t nfrag;        /* non-empty (cell, triangle) fragments              */
   int64_t ncand;        /* sum over bins of candidate owners                  */
   int64_t max_poly;     /* largest cell polygon seen                          */
   int64_t empty_cells;  /* Diracs with zero mass                              */
+  int64_t npairs;       /* (cell, triangle) pairs clipped                     */
+  int64_t ngeneral;     /* cells evaluated by the general (overflow) kernel   */
+  int64_t nbinpath;     /* cells built from bin candidates (not certified by
+                           the site-centric search)                           */
 } ma_eval_info;
 
 ma_status ma_create(int device, ma_engine **out);
@@ -141,6 +146,13 @@ ma_status ma_first_moment(ma_engine *e, const double *w, double *mass, double *m
 ma_status ma_adjoint(ma_engine *e, const double *w, double *dY, double *dval);
 
 ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info);
+
+/* Engine options.  MA_OPT_FORCE_GENERAL (0/1): evaluate every cell with the
+ * general kernel instead of the register/LDS fast path (for testing the
+ * overflow path); MA_OPT_CAP_CAND: initial capacity of the per-bin candidate
+ * lists (grows automatically on overflow). */
+typedef enum { MA_OPT_FORCE_GENERAL = 1, MA_OPT_CAP_CAND = 2 } ma_option;
+ma_status ma_set_option(ma_engine *e, int option, int64_t value);
 void ma_default_solve_opts(ma_solve_opts *o);
 
 #ifdef __cplusplus

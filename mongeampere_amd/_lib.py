@@ -38,8 +38,10 @@ class Stats(ctypes.Structure):
 
 class EvalInfo(ctypes.Structure):
     _fields_ = [("ms_total", F64), ("ms_weights", F64), ("ms_candidates", F64),
-                ("ms_cells", F64), ("ms_assemble", F64), ("nnz", I64), ("nfrag", I64),
-                ("ncand", I64), ("max_poly", I64), ("empty_cells", I64)]
+                ("ms_cells", F64), ("ms_assemble", F64), ("ms_cell_build", F64),
+                ("ms_pairs", F64), ("ms_frag", F64), ("nnz", I64), ("nfrag", I64),
+                ("ncand", I64), ("max_poly", I64), ("empty_cells", I64), ("npairs", I64),
+                ("ngeneral", I64), ("nbinpath", I64)]
 
 
 # (name, restype, argtypes) of every exported entry point of include/ma/ma.h
@@ -58,6 +60,7 @@ SIGNATURES = [
     ("ma_adjoint", I32, [P, P, P, P]),
     ("ma_get_eval_info", I32, [P, P]),
     ("ma_default_solve_opts", None, [P]),
+    ("ma_set_option", I32, [P, I32, I64]),
 ]
 
 _LIB = None

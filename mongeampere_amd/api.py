@@ -85,6 +85,11 @@ class Engine:
             msg = self._L.ma_last_error(self._h)
             raise EngineError(st, msg.decode() if msg else "")
 
+    def set_option(self, name, value):
+        key = {"force_general": 1, "cap_cand": 2}[name]
+        self._check(self._L.ma_set_option(self._h, key, int(value)))
+        return self
+
     # ---- inputs -----------------------------------------------------------
     def set_mesh(self, mesh):
         vxy = np.ascontiguousarray(mesh.vxy, np.float64)

@@ -77,6 +77,9 @@ struct EvalBuffers {
   DBuf<unsigned long long> counters;  // [0] fragments, [1] candidates, [2] max poly, [3] empty
   DBuf<int32_t> scan_tmp;
   DBuf<double> red_tmp;
+  // register/LDS fast path
+  DBuf<double> px, py, eacc, bbox;
+  DBuf<int32_t> plab, ehit, pn, tcount, poff, pair_cell, pair_tri, glist;
 };
 
 struct SolverBuffers {
@@ -89,7 +92,7 @@ struct SolverBuffers {
 struct ma_engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[8] = {};
+  hipEvent_t ev[10] = {};
   std::string err;
 
   // mesh
@@ -112,6 +115,7 @@ struct ma_engine {
   bool grid_valid = false;     // triangle bins built for the current grid
 
   int cap_cand = 48;
+  int force_general = 0;  // MA_OPT_FORCE_GENERAL
   int cap_row = ma::kMaxPoly + 1;
 
   ma::EvalBuffers eb;

@@ -748,6 +748,21 @@ ma_status ma_adjoint(ma_engine *e, const double *w, double *dY, double *dval) {
   return adjoint(e, dY, dval);
 }
 
+ma_status ma_set_option(ma_engine *e, int option, int64_t value) {
+  if (!e) return MA_BAD_INPUT;
+  switch (option) {
+    case MA_OPT_FORCE_GENERAL:
+      e->force_general = value != 0;
+      return MA_OK;
+    case MA_OPT_CAP_CAND:
+      if (value < 4 || value > 4096) return MA_BAD_INPUT;
+      e->cap_cand = (int)value;
+      return MA_OK;
+    default:
+      return MA_BAD_INPUT;
+  }
+}
+
 ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info) {
   if (!e || !info) return MA_BAD_INPUT;
   *info = e->info;

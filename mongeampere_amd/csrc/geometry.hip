@@ -13,41 +13,9 @@
 #include <climits>
 #include <cmath>
 
-#include "engine.h"
+#include "geom_common.h"
 
 namespace ma {
-
-constexpr int kBlock = 128;
-constexpr double kInf = HUGE_VAL;
-
-__host__ __device__ inline uint32_t part1by1(uint32_t x) {
-  x &= 0x0000ffffu;
-  x = (x | (x << 8)) & 0x00ff00ffu;
-  x = (x | (x << 4)) & 0x0f0f0f0fu;
-  x = (x | (x << 2)) & 0x33333333u;
-  x = (x | (x << 1)) & 0x55555555u;
-  return x;
-}
-__host__ __device__ inline uint32_t compact1by1(uint32_t x) {
-  x &= 0x55555555u;
-  x = (x | (x >> 1)) & 0x33333333u;
-  x = (x | (x >> 2)) & 0x0f0f0f0fu;
-  x = (x | (x >> 4)) & 0x00ff00ffu;
-  x = (x | (x >> 8)) & 0x0000ffffu;
-  return x;
-}
-__host__ __device__ inline uint32_t morton2(uint32_t x, uint32_t y) {
-  return part1by1(x) | (part1by1(y) << 1);
-}
-__device__ inline int bin_coord(double v, double o, double h, int G) {
-  double f = floor((v - o) / h);
-  int i = (f < 0.0) ? 0 : (f > (double)(G - 1) ? G - 1 : (int)f);
-  return i;
-}
-
-static inline unsigned nblocks(int64_t n, int b = kBlock) {
-  return (unsigned)((n + b - 1) / b);
-}
 
 // ---------------------------------------------------------------------------
 // Exclusive prefix sum of int32 (wave64 shuffles, 1024 elements per block).
@@ -266,13 +234,6 @@ hipError_t build_dirac_grid(ma_engine *e, const double *d_X, int layout) {
 // ---------------------------------------------------------------------------
 // Triangle bins: every triangle is listed in each bin its bounding box meets.
 // ---------------------------------------------------------------------------
-__device__ inline void tri_bbox(const TriData &t, double &x0, double &y0, double &x1,
-                                double &y1) {
-  x0 = fmin(t.x[0], fmin(t.x[1], t.x[2]));
-  x1 = fmax(t.x[0], fmax(t.x[1], t.x[2]));
-  y0 = fmin(t.y[0], fmin(t.y[1], t.y[2]));
-  y1 = fmax(t.y[0], fmax(t.y[1], t.y[2]));
-}
 template <bool FILL>
 __global__ void k_tbin(const TriData *tri, int64_t nt, Grid g, int32_t *cnt_or_cursor,
                        int32_t *items) {
@@ -506,8 +467,10 @@ __global__ void k_rect_init(int32_t *r, int64_t n) {
 
 // Bin rectangle of every Dirac: bounding box (in bin coordinates) of the bins
 // listing it as a candidate.
+// Only Diracs marked pn[q] < 0 (uncertified by the site-centric search, or
+// overflowing it) are needed.
 __global__ void k_rect(Grid g, const int32_t *tbin_start, const int32_t *cand_cnt,
-                       const int32_t *cand, int cap, int32_t *rect) {
+                       const int32_t *cand, int cap, const int32_t *pn, int32_t *rect) {
   const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (b >= g.nbins) return;
   if (tbin_start[b + 1] == tbin_start[b]) return;
@@ -515,6 +478,7 @@ __global__ void k_rect(Grid g, const int32_t *tbin_start, const int32_t *cand_cn
   const int c = cand_cnt[b];
   for (int i = 0; i < c; i++) {
     int32_t q = cand[b * (int64_t)cap + i];
+    if (pn[q] >= 0) continue;
     atomicMin(&rect[4 * q + 0], bx);
     atomicMin(&rect[4 * q + 1], by);
     atomicMax(&rect[4 * q + 2], bx);
@@ -580,22 +544,24 @@ __device__ inline bool clip_cell(Poly &P, Poly &Q, double Yx, double Yy, double 
   return true;
 }
 
-// Clip a fragment by the closed half-plane left of (ax,ay)->(bx,by).  A
-// polygon with no vertex strictly inside becomes empty (so a cell touching a
-// triangle only along an edge or at a point yields no fragment, as in the
-// reference, where such a triangle is clipped away entirely).
+// Clip a fragment by the closed half-plane left of (ax,ay)->(bx,by).
+// Degenerate (zero-area) results are discarded by the caller.
 __device__ inline bool clip_tri_edge(const Poly &P, Poly &Q, double ax, double ay, double bx,
                                      double by) {
   const int n = P.n;
   double s[kMaxPoly];
   const double ex = bx - ax, ey = by - ay;
-  int nstrict = 0, nin = 0;
+  const double elen = sqrt(ex * ex + ey * ey);
+  int nin = 0;
   for (int i = 0; i < n; i++) {
-    s[i] = ex * (P.y[i] - ay) - ey * (P.x[i] - ax);
-    nstrict += s[i] > 0.0;
+    const double dx = P.x[i] - ax, dy = P.y[i] - ay;
+    s[i] = ex * dy - ey * dx;
+    // within rounding of the line: on it (see Stage::side)
+    if (fabs(s[i]) <= kSnap * elen * (fabs(P.x[i]) + fabs(P.y[i]) + fabs(dx) + fabs(dy)))
+      s[i] = 0.0;
     nin += s[i] >= 0.0;
   }
-  if (nstrict == 0) {
+  if (nin == 0) {
     Q.n = 0;
     return true;
   }
@@ -636,31 +602,12 @@ __device__ inline bool clip_tri_edge(const Poly &P, Poly &Q, double ax, double a
   return true;
 }
 
-// Albrecht-Collatz 6-point rule (include/MA/quadrature.hpp:23-42) of
-// rho(P) * g(P) over triangle (a, b, c), rho(P) = r0 + ga.P, local frame.
-struct AC6 {
-  double px[6], py[6];
-};
-__device__ inline void ac6_points(double ax, double ay, double bx, double by, double cx,
-                                  double cy, AC6 &q) {
-  const double ux = bx - ax, uy = by - ay, vx = cx - ax, vy = cy - ay;
-  const double h = 0.5, s6 = 1.0 / 6.0, t3 = 2.0 / 3.0;
-  q.px[0] = ax + h * ux + h * vx;
-  q.py[0] = ay + h * uy + h * vy;
-  q.px[1] = ax + h * ux;
-  q.py[1] = ay + h * uy;
-  q.px[2] = ax + h * vx;
-  q.py[2] = ay + h * vy;
-  q.px[3] = ax + s6 * ux + t3 * vx;
-  q.py[3] = ay + s6 * uy + t3 * vy;
-  q.px[4] = ax + s6 * vx + t3 * ux;
-  q.py[4] = ay + s6 * vy + t3 * uy;
-  q.px[5] = ax + s6 * ux + s6 * vx;
-  q.py[5] = ay + s6 * uy + s6 * vy;
-}
-
+// General per-Dirac kernel (polygons up to kMaxPoly vertices in private
+// memory) for the cells listed in `list`: the rare cells that overflow the
+// register/LDS fast path.
 template <int MODE>  // 0: Kantorovich, 1: first moments
-__global__ void __launch_bounds__(kBlock) k_cells(
+__global__ void __launch_bounds__(kBlock) k_cells_general(
+    const int32_t *__restrict__ list, const int32_t *__restrict__ nlist,
     Grid g, int64_t n, const double *__restrict__ pos, const double *__restrict__ w,
     const int32_t *__restrict__ perm, const int32_t *__restrict__ rect,
     const int32_t *__restrict__ cand_cnt, const int32_t *__restrict__ cand, int cap,
@@ -668,8 +615,9 @@ __global__ void __launch_bounds__(kBlock) k_cells(
     const TriData *__restrict__ tri, double *mass_out, double *intg_out, double *fval_part,
     double *mom_out, int32_t *row_cnt, int32_t *row_col, double *row_val, int cap_row,
     int32_t *flags, unsigned long long *counters) {
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= n) return;
+  const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (idx >= *nlist) return;
+  const int64_t k = list[idx];
   const int rx0 = rect[4 * k], ry0 = rect[4 * k + 1], rx1 = rect[4 * k + 2], ry1 = rect[4 * k + 3];
   const double yx = pos[2 * k], yy = pos[2 * k + 1], wk = w[k];
   Poly P, Q, F;
@@ -766,6 +714,21 @@ __global__ void __launch_bounds__(kBlock) k_cells(
           }
           const int m = Q.n;
           if (m < 3) continue;
+          {
+            // fragments of zero area (cell touching the triangle along an
+            // edge or at a point) do not count; see Integrator::positive
+            double ar2 = 0.0, xlo = kInf, xhi = -kInf, ylo = kInf, yhi = -kInf, cm = 0.0;
+            for (int i = 0; i < m; i++) {
+              const int i1 = (i + 1 == m) ? 0 : i + 1;
+              ar2 += Q.x[i] * Q.y[i1] - Q.x[i1] * Q.y[i];
+              xlo = fmin(xlo, Q.x[i]);
+              xhi = fmax(xhi, Q.x[i]);
+              ylo = fmin(ylo, Q.y[i]);
+              yhi = fmax(yhi, Q.y[i]);
+              cm = fmax(cm, fmax(fabs(Q.x[i]), fabs(Q.y[i])));
+            }
+            if (!(0.5 * ar2 > 1e-13 * ((xhi - xlo) + (yhi - ylo)) * cm)) continue;
+          }
           nfrag++;
           const double r0 = t.a * yx + t.b * yy + t.c;  // rho at the Dirac
           // fan from vertex 0 (include/MA/quadrature.hpp:122-143)
@@ -809,6 +772,7 @@ __global__ void __launch_bounds__(kBlock) k_cells(
               if (lab < 0 || P.lab[lab] < 0) continue;
               const int i1 = (i + 1 == m) ? 0 : i + 1;
               const double ex = Q.x[i1] - Q.x[i], ey = Q.y[i1] - Q.y[i];
+              if (ex == 0.0 && ey == 0.0) continue;  // piece from a snapped vertex
               const double mx = 0.5 * (Q.x[i] + Q.x[i1]), my = 0.5 * (Q.y[i] + Q.y[i1]);
               acc[lab] += sqrt(ex * ex + ey * ey) * (r0 + t.a * mx + t.b * my);
               touched |= 1u << lab;
@@ -872,23 +836,9 @@ __global__ void __launch_bounds__(kBlock) k_cells(
     mom_out[2 * k + 1] = momy + yy * mass;
   }
   if (overflow) atomicOr(&flags[0], 1);
-  // block-aggregated counters
-  __shared__ unsigned long long s_frag, s_maxp, s_empty;
-  if (threadIdx.x == 0) {
-    s_frag = 0;
-    s_maxp = 0;
-    s_empty = 0;
-  }
-  __syncthreads();
-  atomicAdd(&s_frag, (unsigned long long)nfrag);
-  atomicMax(&s_maxp, (unsigned long long)np);
-  if (mass <= 0.0) atomicAdd(&s_empty, 1ull);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(&counters[0], s_frag);
-    atomicMax(&counters[2], s_maxp);
-    atomicAdd(&counters[3], s_empty);
-  }
+  atomicAdd(&counters[0], (unsigned long long)nfrag);
+  atomicMax(&counters[2], (unsigned long long)np);
+  atomicAdd(&counters[4], 1ull);
 }
 
 __global__ void k_csr_fill(int64_t n, const int32_t *row_cnt, const int32_t *row_col,
@@ -936,6 +886,39 @@ hipError_t device_sum(ma_engine *e, const double *x, int64_t n, double *d_out) {
 // ---------------------------------------------------------------------------
 // One evaluation.
 // ---------------------------------------------------------------------------
+ma_status run_cells(ma_engine *e, int mode, int32_t *npairs_out, int32_t *ngeneral_out);
+
+// Candidate owners of every domain bin and the bin rectangles of the Diracs
+// marked pn == -2 (see k_candidates, k_rect).
+ma_status run_bin_candidates(ma_engine *e, const int32_t *pn) {
+  const Grid &g = e->grid;
+  const int64_t n = e->n;
+  EvalBuffers &B = e->eb;
+  PyrOff po{};
+  for (int l = 0; l <= g.L; l++) po.o[l] = e->pyr_off[l];
+  for (int attempt = 0;; attempt++) {
+    MA_HIP_TRY(B.cand.ensure((size_t)g.nbins * e->cap_cand));
+    MA_HIP_TRY(hipMemsetAsync(B.flags.p + 1, 0, sizeof(int32_t), e->stream));
+    hipLaunchKernelGGL(k_candidates, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
+                       e->tbin_start.p, e->bin_start.p, e->pos.p, B.w_s.p, B.pyr.p, po,
+                       B.cand_cnt.p, B.cand.p, e->cap_cand, B.owner.p, B.flags.p, B.counters.p);
+    int32_t fl[4];
+    MA_HIP_TRY(hipMemcpyAsync(fl, B.flags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
+    MA_HIP_TRY(hipStreamSynchronize(e->stream));
+    if (!fl[1]) break;
+    if (attempt >= 3) {
+      e->err = "candidate list overflow (cap " + std::to_string(e->cap_cand) + ")";
+      return MA_CAPACITY;
+    }
+    e->cap_cand *= 2;
+  }
+  hipLaunchKernelGGL(k_rect_init, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, B.rect.p, n);
+  hipLaunchKernelGGL(k_rect, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
+                     e->tbin_start.p, B.cand_cnt.p, B.cand.p, e->cap_cand, pn, B.rect.p);
+  MA_HIP_TRY(hipGetLastError());
+  return MA_OK;
+}
+
 ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
   const Grid &g = e->grid;
   const int64_t n = e->n;
@@ -961,14 +944,14 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
   MA_HIP_TRY(B.row_col.ensure((size_t)n * e->cap_row));
   MA_HIP_TRY(B.row_val.ensure((size_t)n * e->cap_row));
   MA_HIP_TRY(B.flags.ensure(4));
-  MA_HIP_TRY(B.counters.ensure(4));
+  MA_HIP_TRY(B.counters.ensure(8));
   MA_HIP_TRY(B.diag.ensure((size_t)n));
   MA_HIP_TRY(B.red_tmp.ensure(2048));
   if (!e->have_eval) {
     hipLaunchKernelGGL(k_fill_i32, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream,
                        B.owner.p, -1, g.nbins);
   }
-  hipEventRecord(e->ev[0], e->stream);
+  MA_HIP_TRY(hipEventRecord(e->ev[0], e->stream));
   hipLaunchKernelGGL(k_pyr_leaf, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream,
                      e->bin_start.p, B.w_s.p, g.nbins, B.pyr.p);
   for (int l = 1; l <= g.L; l++) {
@@ -976,42 +959,32 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
     hipLaunchKernelGGL(k_pyr_up, dim3(nblocks(nh)), dim3(kBlock), 0, e->stream,
                        B.pyr.p + e->pyr_off[l - 1], B.pyr.p + e->pyr_off[l], nh);
   }
-  hipEventRecord(e->ev[1], e->stream);
-  for (int attempt = 0;; attempt++) {
-    MA_HIP_TRY(B.cand.ensure((size_t)g.nbins * e->cap_cand));
-    MA_HIP_TRY(hipMemsetAsync(B.flags.p, 0, sizeof(int32_t) * 4, e->stream));
-    MA_HIP_TRY(hipMemsetAsync(B.counters.p, 0, sizeof(unsigned long long) * 4, e->stream));
-    hipLaunchKernelGGL(k_candidates, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
-                       e->tbin_start.p, e->bin_start.p, e->pos.p, B.w_s.p, B.pyr.p, po,
-                       B.cand_cnt.p, B.cand.p, e->cap_cand, B.owner.p, B.flags.p, B.counters.p);
-    int32_t fl[4];
-    MA_HIP_TRY(hipMemcpyAsync(fl, B.flags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
-    MA_HIP_TRY(hipStreamSynchronize(e->stream));
-    if (!fl[1]) break;
-    if (attempt >= 3) {
-      e->err = "candidate list overflow (cap " + std::to_string(e->cap_cand) + ")";
-      return MA_CAPACITY;
+  MA_HIP_TRY(hipEventRecord(e->ev[1], e->stream));
+  MA_HIP_TRY(hipMemsetAsync(B.flags.p, 0, sizeof(int32_t) * 4, e->stream));
+  MA_HIP_TRY(hipMemsetAsync(B.counters.p, 0, sizeof(unsigned long long) * 8, e->stream));
+  MA_HIP_TRY(hipEventRecord(e->ev[2], e->stream));
+  int32_t npairs = 0, ngen = 0;
+  ma_status cs = run_cells(e, mode, &npairs, &ngen);
+  if (cs != MA_OK) return cs;
+  if (ngen > 0) {
+    // cells that overflowed the fast path
+    if (mode == 0) {
+      hipLaunchKernelGGL(k_cells_general<0>, dim3(nblocks(ngen)), dim3(kBlock), 0, e->stream,
+                         B.glist.p, B.glist.p + n, g, n, e->pos.p, B.w_s.p, e->perm.p, B.rect.p,
+                         B.cand_cnt.p, B.cand.p, e->cap_cand, e->tbin_start.p, e->tbin_items.p,
+                         e->tri.p, B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p,
+                         B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row, B.flags.p,
+                         B.counters.p);
+    } else {
+      hipLaunchKernelGGL(k_cells_general<1>, dim3(nblocks(ngen)), dim3(kBlock), 0, e->stream,
+                         B.glist.p, B.glist.p + n, g, n, e->pos.p, B.w_s.p, e->perm.p, B.rect.p,
+                         B.cand_cnt.p, B.cand.p, e->cap_cand, e->tbin_start.p, e->tbin_items.p,
+                         e->tri.p, B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p,
+                         B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row, B.flags.p,
+                         B.counters.p);
     }
-    e->cap_cand *= 2;
   }
-  hipLaunchKernelGGL(k_rect_init, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, B.rect.p, n);
-  hipLaunchKernelGGL(k_rect, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
-                     e->tbin_start.p, B.cand_cnt.p, B.cand.p, e->cap_cand, B.rect.p);
-  hipEventRecord(e->ev[2], e->stream);
-  if (mode == 0) {
-    hipLaunchKernelGGL(k_cells<0>, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, g, n, e->pos.p,
-                       B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p, B.cand.p, e->cap_cand,
-                       e->tbin_start.p, e->tbin_items.p, e->tri.p, B.mass_s.p, B.intg_s.p,
-                       B.fval_part.p, B.mom_s.p, B.row_cnt.p, B.row_col.p, B.row_val.p,
-                       e->cap_row, B.flags.p, B.counters.p);
-  } else {
-    hipLaunchKernelGGL(k_cells<1>, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, g, n, e->pos.p,
-                       B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p, B.cand.p, e->cap_cand,
-                       e->tbin_start.p, e->tbin_items.p, e->tri.p, B.mass_s.p, B.intg_s.p,
-                       B.fval_part.p, B.mom_s.p, B.row_cnt.p, B.row_col.p, B.row_val.p,
-                       e->cap_row, B.flags.p, B.counters.p);
-  }
-  hipEventRecord(e->ev[3], e->stream);
+  MA_HIP_TRY(hipEventRecord(e->ev[6], e->stream));
   if (mode == 0) {
     MA_HIP_TRY(B.csr_ptr.ensure((size_t)n + 1));
     int32_t zero = 0;
@@ -1030,10 +1003,10 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
     MA_HIP_TRY(device_sum(e, B.fval_part.p, n, B.red_tmp.p + 1024));
     e->nnz = nnz;
   }
-  hipEventRecord(e->ev[4], e->stream);
+  MA_HIP_TRY(hipEventRecord(e->ev[7], e->stream));
   MA_HIP_TRY(hipGetLastError());
   int32_t fl[4];
-  unsigned long long ct[4];
+  unsigned long long ct[8];
   double fv = 0.0;
   MA_HIP_TRY(hipMemcpyAsync(fl, B.flags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
   MA_HIP_TRY(hipMemcpyAsync(ct, B.counters.p, sizeof ct, hipMemcpyDeviceToHost, e->stream));
@@ -1041,20 +1014,23 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
     MA_HIP_TRY(hipMemcpyAsync(&fv, B.red_tmp.p + 1024, sizeof(double), hipMemcpyDeviceToHost,
                               e->stream));
   MA_HIP_TRY(hipStreamSynchronize(e->stream));
-  float t01 = 0, t12 = 0, t23 = 0, t34 = 0;
-  hipEventElapsedTime(&t01, e->ev[0], e->ev[1]);
-  hipEventElapsedTime(&t12, e->ev[1], e->ev[2]);
-  hipEventElapsedTime(&t23, e->ev[2], e->ev[3]);
-  hipEventElapsedTime(&t34, e->ev[3], e->ev[4]);
-  e->info.ms_weights = t01;
-  e->info.ms_candidates = t12;
-  e->info.ms_cells = t23;
-  e->info.ms_assemble = t34;
-  e->info.ms_total = t01 + t12 + t23 + t34;
+  float t[7] = {0, 0, 0, 0, 0, 0, 0}, tc = 0.0f;
+  for (int i = 0; i < 7; i++) MA_HIP_TRY(hipEventElapsedTime(&t[i], e->ev[i], e->ev[i + 1]));
+  if (e->info.nbinpath > 0 || ngen > 0) MA_HIP_TRY(hipEventElapsedTime(&tc, e->ev[8], e->ev[9]));
+  e->info.ms_weights = t[0];
+  e->info.ms_candidates = tc;
+  e->info.ms_cell_build = t[2];
+  e->info.ms_pairs = t[3];
+  e->info.ms_frag = t[4];
+  e->info.ms_cells = t[2] + t[3] + t[4] + t[5];
+  e->info.ms_assemble = t[6];
+  e->info.ms_total = t[0] + t[1] + t[2] + t[3] + t[4] + t[5] + t[6];
   e->info.nfrag = (int64_t)ct[0];
   e->info.ncand = (int64_t)ct[1];
   e->info.max_poly = (int64_t)ct[2];
   e->info.empty_cells = (int64_t)ct[3];
+  e->info.npairs = npairs;
+  e->info.ngeneral = ngen;
   e->info.nnz = e->nnz;
   if (fl[0]) {
     e->err = "cell polygon exceeded " + std::to_string(kMaxPoly) + " vertices";

@@ -1,0 +1,51 @@
+"""The C ABI library loads and exports every entry point include/ma/ma.h
+declares; the Python binding covers all of them.  No GPU calls."""
+import ctypes
+import os
+import re
+
+from conftest import ROOT
+from mongeampere_amd import _lib
+
+HEADER = os.path.join(ROOT, "include", "ma", "ma.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ma_[a-z_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ("ma_create", "ma_destroy", "ma_set_mesh", "ma_set_diracs", "ma_kantorovich",
+                 "ma_hessian_copy", "ma_ot_solve", "ma_solve_laplacian", "ma_first_moment",
+                 "ma_adjoint"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    bound = {name for name, _, _ in _lib.SIGNATURES}
+    assert set(declared_functions()) == bound
+
+
+def test_struct_layouts_match_header():
+    src = open(HEADER).read()
+    for cname, py in (("ma_solve_opts", _lib.SolveOpts), ("ma_stats", _lib.Stats),
+                      ("ma_eval_info", _lib.EvalInfo)):
+        body = re.search(r"typedef struct \{([^{}]*?)\}\s*" + cname + ";", src, re.S).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        fields = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            typ, names = decl.split(None, 1)
+            fields += [n.strip() for n in names.split(",")]
+        assert fields == [f for f, _ in py._fields_], cname

@@ -25,12 +25,18 @@ def _pattern(H, rel=0.0):
     return set(zip(*np.nonzero(np.abs(A) > thr)))
 
 
+@pytest.mark.parametrize("path", ["fast", "general"])
 @pytest.mark.parametrize("name", GOLDEN_CASES)
-def test_golden_exact(engine, name):
+def test_golden_exact(engine, name, path):
     c = load_golden(name)
-    print(f"case {name}", flush=True)
+    print(f"case {name} path {path}", flush=True)
+    engine.set_option("force_general", path == "general")
     engine.set_mesh(c["mesh"]).set_diracs(c["X"])
-    fval, g, H = engine.kantorovich(c["w"])
+    try:
+        fval, g, H = engine.kantorovich(c["w"])
+        m, mom = engine.first_moment(c["w"])
+    finally:
+        engine.set_option("force_general", 0)
     assert _rel(g, c["mass"]) <= 1e-12
     assert abs(fval - c["fval"]) <= 1e-12 * max(abs(c["fval"]), 1e-300)
     if c["H"].any():
@@ -39,7 +45,6 @@ def test_golden_exact(engine, name):
         assert H.nnz == 0
     # adjacency identical except for entries at the level of exact ties
     assert _pattern(H, 1e-12) == _pattern(c["H"], 1e-12)
-    m, mom = engine.first_moment(c["w"])
     assert _rel(m, c["mass"]) <= 1e-12
     assert np.abs(mom - c["mom"]).max() <= 1e-12 * max(np.abs(c["mom"]).max(), 1e-300)
 
@@ -59,12 +64,17 @@ def random_problem(seed):
     return mesh, X, w
 
 
+@pytest.mark.parametrize("path", ["fast", "general"])
 @pytest.mark.parametrize("seed", range(12))
-def test_random_vs_oracle(engine, seed):
+def test_random_vs_oracle(engine, seed, path):
     mesh, X, w = random_problem(seed)
-    print(f"seed {seed} N={X.shape[0]} T={mesh.tri.shape[0]}", flush=True)
+    print(f"seed {seed} N={X.shape[0]} T={mesh.tri.shape[0]} path {path}", flush=True)
+    engine.set_option("force_general", path == "general")
     engine.set_mesh(mesh).set_diracs(X)
-    fval, g, H = engine.kantorovich(w)
+    try:
+        fval, g, H = engine.kantorovich(w)
+    finally:
+        engine.set_option("force_general", 0)
     of, og, oH = oracle.kantorovich(mesh, X, w)
     assert _rel(g, og) <= TOL
     assert abs(fval - of) <= TOL * max(abs(of), 1e-300)
