@@ -58,6 +58,7 @@ typedef struct {
   double cg_tol;      /* relative residual of the linear solves (0 = auto)   */
   int cg_maxiter;     /* default 20000                                        */
   int verbose;        /* print one line per Newton step to stderr             */
+  int preconditioner; /* 0: aggregation multigrid (default), 1: Jacobi        */
 } ma_solve_opts;
 
 typedef struct {

@@ -27,7 +27,7 @@ MA_OK, MA_EMPTY_CELL, MA_NOT_CONVERGED, MA_LINESEARCH_FAIL = 0, 2, 4, 5
 class SolveOpts(ctypes.Structure):
     _fields_ = [("precision", I32), ("stop_rule", I32), ("eps_g", F64), ("tol", F64),
                 ("maxiter", I32), ("alpha_min", F64), ("cg_tol", F64), ("cg_maxiter", I32),
-                ("verbose", I32)]
+                ("verbose", I32), ("preconditioner", I32)]
 
 
 class Stats(ctypes.Structure):

@@ -160,7 +160,7 @@ class Engine:
 
     def ot_solve(self, nu, x=None, eps_g=1e-7, maxiter=100, verbose=False, stop="l2",
                  tol=1e-8, precision="fp64", cg_tol=0.0, cg_maxiter=20000,
-                 alpha_min=2.0 ** -30, raise_on_error=True):
+                 alpha_min=2.0 ** -30, raise_on_error=True, preconditioner="amg"):
         nu = np.ascontiguousarray(nu, np.float64)
         init = x is not None and len(x) == self.n
         w = np.ascontiguousarray(x, np.float64).copy() if init else np.zeros(self.n)
@@ -175,6 +175,7 @@ class Engine:
         o.cg_tol = cg_tol
         o.cg_maxiter = int(cg_maxiter)
         o.verbose = int(bool(verbose))
+        o.preconditioner = {"amg": 0, "jacobi": 1}[preconditioner]
         s = _lib.Stats()
         st = self._L.ma_ot_solve(self._h, _ptr(nu), _ptr(w), int(init), ctypes.byref(o),
                                  ctypes.byref(s))

@@ -1,0 +1,318 @@
+// Aggregation multigrid preconditioner for the gauge-fixed Hessian (a
+// weighted graph Laplacian of the power diagram) and the conjugate gradient
+// that uses it.  Replaces the sparse direct factorisation of
+// solve_laplacian_matrix (include/MA/optimal_transport.hpp:41-87).
+//
+// Hierarchy: aggregates are runs of kAgg consecutive unknowns in Morton
+// order (spatially compact groups of Diracs), prolongation is piecewise
+// constant, coarse operators are Galerkin products P^T A P.  Because the
+// aggregate map i -> i / kAgg is monotone, a coarse row is the merge of
+// kAgg sorted fine rows.  Cycle: V(2,2) with damped Jacobi (omega = 2/3),
+// coarse correction scaled by 1.5 (over-correction for piecewise-constant
+// prolongation), dense Cholesky at the coarsest level.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "geom_common.h"
+
+namespace ma {
+
+constexpr int kAgg = 4;
+constexpr int kCoarsest = 192;   // dense solve at or below this size
+constexpr double kOmega = 2.0 / 3.0;
+constexpr double kCorr = 1.5;
+constexpr int kVB2 = 256;
+
+// ---------------------------------------------------------------------------
+// Setup
+// ---------------------------------------------------------------------------
+
+// Level 0: the Hessian with row and column gi removed (identity there), so
+// the system is SPD and the solution keeps x[gi] = 0.
+__global__ void k_amg_level0(int64_t n, const int32_t *ptr, const int32_t *col,
+                             const double *val, int32_t gi, double *out, double *dinv) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = 0.0;
+  for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) {
+    const int32_t j = col[q];
+    double v = val[q];
+    if (i == gi || j == gi) v = (i == j) ? 1.0 : 0.0;
+    out[q] = v;
+    if (j == i) d = v;
+  }
+  if (i == gi) d = 1.0;
+  dinv[i] = d > 0.0 ? 1.0 / d : 0.0;
+}
+
+// Merge of the kAgg sorted fine rows of coarse row I (columns mapped j/kAgg).
+template <bool FILL>
+__global__ void k_amg_coarse(int64_t nc, int64_t n, const int32_t *ptr, const int32_t *col,
+                             const double *val, int32_t *cnt_or_ptr, int32_t *ccol,
+                             double *cval, double *cdinv) {
+  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  int32_t pos[kAgg], end[kAgg];
+#pragma unroll
+  for (int r = 0; r < kAgg; r++) {
+    const int64_t i = I * kAgg + r;
+    pos[r] = (i < n) ? ptr[i] : 0;
+    end[r] = (i < n) ? ptr[i + 1] : 0;
+  }
+  int32_t o = FILL ? cnt_or_ptr[I] : 0;
+  int32_t cnt = 0;
+  double diag = 0.0;
+  while (true) {
+    int32_t cmin = INT32_MAX;
+#pragma unroll
+    for (int r = 0; r < kAgg; r++)
+      if (pos[r] < end[r]) cmin = min(cmin, col[pos[r]] / kAgg);
+    if (cmin == INT32_MAX) break;
+    double s = 0.0;
+#pragma unroll
+    for (int r = 0; r < kAgg; r++)
+      while (pos[r] < end[r] && col[pos[r]] / kAgg == cmin) {
+        s += val[pos[r]];
+        pos[r]++;
+      }
+    if (FILL) {
+      ccol[o + cnt] = cmin;
+      cval[o + cnt] = s;
+      if (cmin == I) diag = s;
+    }
+    cnt++;
+  }
+  if (FILL)
+    cdinv[I] = diag > 0.0 ? 1.0 / diag : 0.0;
+  else
+    cnt_or_ptr[I] = cnt;
+}
+
+// Dense coarsest matrix and its Cholesky factor (one workgroup).
+__global__ void __launch_bounds__(256) k_amg_dense_chol(int64_t n, const int32_t *ptr,
+                                                        const int32_t *col, const double *val,
+                                                        double *Lf) {
+  for (int64_t q = threadIdx.x; q < n * n; q += blockDim.x) Lf[q] = 0.0;
+  __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x)
+    for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) Lf[i * n + col[q]] = val[q];
+  __syncthreads();
+  // right-looking Cholesky, lower triangle, row-major
+  for (int64_t k = 0; k < n; k++) {
+    if (threadIdx.x == 0) {
+      const double d = Lf[k * n + k];
+      Lf[k * n + k] = d > 0.0 ? sqrt(d) : 1.0;
+    }
+    __syncthreads();
+    const double lkk = Lf[k * n + k];
+    for (int64_t i = k + 1 + threadIdx.x; i < n; i += blockDim.x) Lf[i * n + k] /= lkk;
+    __syncthreads();
+    for (int64_t q = threadIdx.x; q < (n - k - 1) * (n - k - 1); q += blockDim.x) {
+      const int64_t i = k + 1 + q / (n - k - 1), j = k + 1 + q % (n - k - 1);
+      if (j <= i) Lf[i * n + j] -= Lf[i * n + k] * Lf[j * n + k];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cycle kernels (all return at once when the conjugate gradient has
+// converged: done[0] != 0)
+// ---------------------------------------------------------------------------
+// x = omega * b / D   (first sweep from x = 0)
+__global__ void k_amg_smooth0(int64_t n, const double *dinv, const double *b, double *x,
+                              const double *done) {
+  if (*done != 0.0) return;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = kOmega * b[i] * dinv[i];
+}
+// y = x + omega (b - A x) / D
+__global__ void k_amg_smooth(int64_t n, const int32_t *ptr, const int32_t *col,
+                             const double *val, const double *dinv, const double *b,
+                             const double *x, double *y, const double *done) {
+  if (*done != 0.0) return;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ax = 0.0;
+  for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) ax += val[q] * x[col[q]];
+  y[i] = x[i] + kOmega * (b[i] - ax) * dinv[i];
+}
+// bc[I] = sum over the aggregate of (b - A x)
+__global__ void k_amg_restrict(int64_t nc, int64_t n, const int32_t *ptr, const int32_t *col,
+                               const double *val, const double *b, const double *x,
+                               double *bc, const double *done) {
+  if (*done != 0.0) return;
+  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (I >= nc) return;
+  double s = 0.0;
+  for (int r = 0; r < kAgg; r++) {
+    const int64_t i = I * kAgg + r;
+    if (i >= n) break;
+    double ax = 0.0;
+    for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) ax += val[q] * x[col[q]];
+    s += b[i] - ax;
+  }
+  bc[I] = s;
+}
+// x += corr * xc[i / kAgg]
+__global__ void k_amg_prolong(int64_t n, const double *xc, double *x, const double *done) {
+  if (*done != 0.0) return;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] += kCorr * xc[i / kAgg];
+}
+// coarsest: x = L^-T L^-1 b (one workgroup)
+__global__ void __launch_bounds__(256) k_amg_dense_solve(int64_t n, const double *Lf,
+                                                         const double *b, double *x,
+                                                         const double *done) {
+  if (*done != 0.0) return;
+  __shared__ double y[kCoarsest];
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y[i] = b[i];
+  __syncthreads();
+  for (int64_t k = 0; k < n; k++) {
+    if (threadIdx.x == 0) y[k] /= Lf[k * n + k];
+    __syncthreads();
+    for (int64_t i = k + 1 + threadIdx.x; i < n; i += blockDim.x) y[i] -= Lf[i * n + k] * y[k];
+    __syncthreads();
+  }
+  for (int64_t k = n - 1; k >= 0; k--) {
+    if (threadIdx.x == 0) y[k] /= Lf[k * n + k];
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < k; i += blockDim.x) y[i] -= Lf[k * n + i] * y[k];
+    __syncthreads();
+  }
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = y[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+struct AmgLevel {
+  int64_t n = 0;
+  DBuf<int32_t> ptr, col;
+  DBuf<double> val, dinv, x, y, b;
+};
+
+struct Amg {
+  std::vector<AmgLevel> lv;
+  DBuf<double> dense;
+  int nlev = 0;
+  ~Amg() {
+    for (auto &l : lv) {
+      l.ptr.release(); l.col.release(); l.val.release(); l.dinv.release();
+      l.x.release(); l.y.release(); l.b.release();
+    }
+    dense.release();
+  }
+};
+
+static inline unsigned nb2(int64_t n) { return (unsigned)((n + kVB2 - 1) / kVB2); }
+
+Amg *amg_create() { return new Amg(); }
+void amg_destroy(Amg *a) { delete a; }
+const double *amg_level0_values(const Amg &a) { return a.lv[0].val.p; }
+const double *amg_level0_dinv(const Amg &a) { return a.lv[0].dinv.p; }
+
+// Builds the hierarchy from the Morton-order CSR of the last evaluation
+// (max_levels = 1: the gauge-fixed level-0 matrix only).
+ma_status amg_setup(ma_engine *e, Amg &A, int32_t gi, int max_levels) {
+  const int64_t n = e->n;
+  EvalBuffers &B = e->eb;
+  A.lv.clear();
+  A.lv.reserve(24);
+  A.lv.emplace_back();
+  AmgLevel &L0 = A.lv.back();
+  L0.n = n;
+  MA_HIP_TRY(L0.val.ensure((size_t)e->nnz + 1));
+  MA_HIP_TRY(L0.dinv.ensure((size_t)n));
+  MA_HIP_TRY(L0.x.ensure((size_t)n));
+  MA_HIP_TRY(L0.y.ensure((size_t)n));
+  hipLaunchKernelGGL(k_amg_level0, dim3(nb2(n)), dim3(kVB2), 0, e->stream, n, B.csr_ptr.p,
+                     B.csr_col.p, B.csr_val.p, gi, L0.val.p, L0.dinv.p);
+  if (max_levels == 1) {
+    A.nlev = 1;
+    MA_HIP_TRY(hipGetLastError());
+    return MA_OK;
+  }
+  while (A.lv.back().n > kCoarsest) {
+    const size_t l = A.lv.size() - 1;
+    const int64_t nf = A.lv[l].n, nc = (nf + kAgg - 1) / kAgg;
+    const int32_t *fptr = (l == 0) ? B.csr_ptr.p : A.lv[l].ptr.p;
+    const int32_t *fcol = (l == 0) ? B.csr_col.p : A.lv[l].col.p;
+    const double *fval = A.lv[l].val.p;
+    A.lv.emplace_back();
+    AmgLevel &C = A.lv.back();
+    C.n = nc;
+    MA_HIP_TRY(C.ptr.ensure((size_t)nc + 1));
+    MA_HIP_TRY(C.dinv.ensure((size_t)nc));
+    MA_HIP_TRY(C.x.ensure((size_t)nc));
+    MA_HIP_TRY(C.y.ensure((size_t)nc));
+    MA_HIP_TRY(C.b.ensure((size_t)nc));
+    hipLaunchKernelGGL(k_amg_coarse<false>, dim3(nb2(nc)), dim3(kVB2), 0, e->stream, nc, nf,
+                       fptr, fcol, fval, C.ptr.p, nullptr, nullptr, nullptr);
+    const int32_t zero = 0;
+    MA_HIP_TRY(hipMemcpyAsync(C.ptr.p + nc, &zero, sizeof(int32_t), hipMemcpyHostToDevice,
+                              e->stream));
+    MA_HIP_TRY(exclusive_scan(e, C.ptr.p, C.ptr.p, nc + 1));
+    int32_t nnzc = 0;
+    MA_HIP_TRY(hipMemcpyAsync(&nnzc, C.ptr.p + nc, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              e->stream));
+    MA_HIP_TRY(hipStreamSynchronize(e->stream));
+    MA_HIP_TRY(C.col.ensure((size_t)nnzc + 1));
+    MA_HIP_TRY(C.val.ensure((size_t)nnzc + 1));
+    hipLaunchKernelGGL(k_amg_coarse<true>, dim3(nb2(nc)), dim3(kVB2), 0, e->stream, nc, nf,
+                       fptr, fcol, fval, C.ptr.p, C.col.p, C.val.p, C.dinv.p);
+  }
+  const AmgLevel &Lc = A.lv.back();
+  const int32_t *cptr = (A.lv.size() == 1) ? B.csr_ptr.p : Lc.ptr.p;
+  const int32_t *ccol = (A.lv.size() == 1) ? B.csr_col.p : Lc.col.p;
+  MA_HIP_TRY(A.dense.ensure((size_t)(Lc.n * Lc.n)));
+  hipLaunchKernelGGL(k_amg_dense_chol, dim3(1), dim3(256), 0, e->stream, Lc.n, cptr, ccol,
+                     Lc.val.p, A.dense.p);
+  A.nlev = (int)A.lv.size();
+  MA_HIP_TRY(hipGetLastError());
+  return MA_OK;
+}
+
+// z = M^-1 r (one V-cycle); done: device flag, cycles are skipped once set.
+hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const double *done) {
+  EvalBuffers &B = e->eb;
+  const int nl = A.nlev;
+  auto ptr_of = [&](int l) { return l == 0 ? B.csr_ptr.p : A.lv[l].ptr.p; };
+  auto col_of = [&](int l) { return l == 0 ? B.csr_col.p : A.lv[l].col.p; };
+  // downward
+  for (int l = 0; l < nl - 1; l++) {
+    AmgLevel &L = A.lv[l];
+    const double *b = (l == 0) ? r : L.b.p;
+    double *x = (l == 0) ? z : L.x.p;
+    hipLaunchKernelGGL(k_amg_smooth0, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, L.dinv.p, b,
+                       L.y.p, done);
+    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, ptr_of(l),
+                       col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done);
+    hipLaunchKernelGGL(k_amg_restrict, dim3(nb2(A.lv[l + 1].n)), dim3(kVB2), 0, e->stream,
+                       A.lv[l + 1].n, L.n, ptr_of(l), col_of(l), L.val.p, b, x,
+                       A.lv[l + 1].b.p, done);
+  }
+  {
+    AmgLevel &Lc = A.lv[nl - 1];
+    const double *b = (nl == 1) ? r : Lc.b.p;
+    double *x = (nl == 1) ? z : Lc.x.p;
+    hipLaunchKernelGGL(k_amg_dense_solve, dim3(1), dim3(256), 0, e->stream, Lc.n, A.dense.p, b,
+                       x, done);
+  }
+  // upward
+  for (int l = nl - 2; l >= 0; l--) {
+    AmgLevel &L = A.lv[l];
+    const double *b = (l == 0) ? r : L.b.p;
+    double *x = (l == 0) ? z : L.x.p;
+    hipLaunchKernelGGL(k_amg_prolong, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n,
+                       A.lv[l + 1].x.p, x, done);
+    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, ptr_of(l),
+                       col_of(l), L.val.p, L.dinv.p, b, x, L.y.p, done);
+    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, ptr_of(l),
+                       col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace ma

@@ -11,6 +11,8 @@
 
 namespace ma {
 
+struct Amg;  // multigrid hierarchy (amg.hip)
+
 // ---------------------------------------------------------------------------
 // Device buffer (owned, grows on demand, never shrinks until destroyed).
 // ---------------------------------------------------------------------------
@@ -114,6 +116,7 @@ struct ma_engine {
   ma::DBuf<int32_t> tbin_start, tbin_items;
   bool grid_valid = false;     // triangle bins built for the current grid
 
+  ma::Amg *amg = nullptr;
   int cap_cand = 48;
   int force_general = 0;  // MA_OPT_FORCE_GENERAL
   int cap_row = ma::kMaxPoly + 1;
@@ -142,6 +145,13 @@ hipError_t from_morton(ma_engine *e, const double *src_morton, double *dst_calle
 hipError_t exclusive_scan(ma_engine *e, const int32_t *in, int32_t *out, int64_t n);
 hipError_t csr_to_caller(ma_engine *e, int32_t *rowptr, int32_t *col, double *val);
 ma_status adjoint(ma_engine *e, double *dY_caller, double *dval);
+// Multigrid preconditioner (amg.hip)
+Amg *amg_create();
+void amg_destroy(Amg *a);
+ma_status amg_setup(ma_engine *e, Amg &A, int32_t gi, int max_levels);
+hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const double *done);
+const double *amg_level0_values(const Amg &a);
+const double *amg_level0_dinv(const Amg &a);
 
 }  // namespace ma
 

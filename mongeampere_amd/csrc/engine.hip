@@ -77,52 +77,67 @@ __device__ inline double reduce_partials(const double *part, double *s) {
   return block_sum(v, s);
 }
 
-// y = A x on the gauge-fixed system (row/column gi removed; y[gi] = 0),
-// partial sums of x.y.
+// The conjugate gradient runs on A0 = the Hessian with row and column gi
+// replaced by the identity (symmetric positive definite for a connected
+// diagram); with b[gi] = 0 its solution has x[gi] = 0, the gauge of
+// include/MA/optimal_transport.hpp:82-86.  Scalars live on the device:
+// scal = [rz (ping-pong, 2), rr, -, bb, done].
+
+// y = A0 x, partial sums of x.y
 __global__ void __launch_bounds__(kVB) k_spmv_dot(int64_t n, const int32_t *ptr,
                                                   const int32_t *col, const double *val,
-                                                  const double *x, double *y, int32_t gi,
-                                                  double *part) {
+                                                  const double *x, double *y, double *part,
+                                                  const double *scal) {
   __shared__ double s[kVB];
+  if (scal[5] != 0.0) return;
   double acc = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
     double v = 0.0;
-    if (i != gi) {
-      for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) {
-        const int32_t j = col[q];
-        if (j != gi) v += val[q] * x[j];
-      }
-    }
+    for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) v += val[q] * x[col[q]];
     y[i] = v;
     acc += x[i] * v;
   }
   acc = block_sum(acc, s);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
-
-// scal layout: [0..1] rz (ping-pong), [2] rr, [3] pAp, [4] bb
-__global__ void __launch_bounds__(kVB) k_cg_init(int64_t n, const double *b, const double *diag,
-                                                 int32_t gi, double *x, double *r, double *z,
-                                                 double *p, double *part_rz, double *part_rr) {
+// x = 0, r = b (r[gi] = 0), partial sums of r.r; clears the done flag
+__global__ void __launch_bounds__(kVB) k_cg_begin(int64_t n, const double *b, int32_t gi,
+                                                  double *x, double *r, double *part_rr,
+                                                  double *scal) {
   __shared__ double s[kVB];
-  double rz = 0.0, rr = 0.0;
+  double rr = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
     const double ri = (i == gi) ? 0.0 : b[i];
-    const double di = diag[i];
-    const double zi = (di > 0.0) ? ri / di : 0.0;
     x[i] = 0.0;
     r[i] = ri;
-    z[i] = zi;
-    p[i] = zi;
-    rz += ri * zi;
     rr += ri * ri;
   }
-  rz = block_sum(rz, s);
   rr = block_sum(rr, s);
-  if (threadIdx.x == 0) {
-    part_rz[blockIdx.x] = rz;
-    part_rr[blockIdx.x] = rr;
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = rr;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[5] = 0.0;
+    scal[6] = 0.0;
   }
+}
+// z = r / D
+__global__ void k_jacobi(int64_t n, const double *dinv, const double *r, double *z,
+                         const double *scal) {
+  if (scal[5] != 0.0) return;
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) z[i] = r[i] * dinv[i];
+}
+// partial sums of r.z; optionally p = z
+__global__ void __launch_bounds__(kVB) k_dot_rz(int64_t n, const double *r, const double *z,
+                                                double *p, double *part, const double *scal) {
+  __shared__ double s[kVB];
+  if (scal[5] != 0.0) return;
+  double a = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
+    a += r[i] * z[i];
+    if (p) p[i] = z[i];
+  }
+  a = block_sum(a, s);
+  if (threadIdx.x == 0) part[blockIdx.x] = a;
 }
 __global__ void __launch_bounds__(kVB) k_cg_scal_init(const double *part_rz, const double *part_rr,
                                                       double *scal) {
@@ -133,39 +148,30 @@ __global__ void __launch_bounds__(kVB) k_cg_scal_init(const double *part_rz, con
     scal[0] = rz;
     scal[2] = rr;
     scal[4] = rr;
-    scal[5] = 0.0;  // done flag
+    if (rr == 0.0) scal[5] = 1.0;
   }
 }
-// x += alpha p, r -= alpha Ap, z = r / diag; partials of r.z and r.r
+// x += alpha p, r -= alpha Ap; partial sums of r.r
 __global__ void __launch_bounds__(kVB) k_cg_update(int64_t n, int parity, const double *part_pap,
-                                                   double *scal, const double *diag, int32_t gi,
-                                                   double *x, double *r, double *z,
+                                                   double *scal, double *x, double *r,
                                                    const double *p, const double *Ap,
-                                                   double *part_rz, double *part_rr) {
+                                                   double *part_rr) {
   __shared__ double s[kVB];
   if (scal[5] != 0.0) return;
   const double pap = reduce_partials(part_pap, s);
   const double rz = scal[parity];
   const double alpha = (pap > 0.0) ? rz / pap : 0.0;
-  double a = 0.0, c = 0.0;
+  double c = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
-    if (i == gi) continue;
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * Ap[i];
     r[i] = ri;
-    const double di = diag[i];
-    const double zi = (di > 0.0) ? ri / di : 0.0;
-    z[i] = zi;
-    a += ri * zi;
     c += ri * ri;
   }
-  a = block_sum(a, s);
   c = block_sum(c, s);
-  if (threadIdx.x == 0) {
-    part_rz[blockIdx.x] = a;
-    part_rr[blockIdx.x] = c;
-  }
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = c;
 }
+__global__ void k_set_zero_at(double *x, int32_t i) { x[i] = 0.0; }
 // p = z + beta p; writes the new r.z into the other parity slot
 __global__ void __launch_bounds__(kVB) k_cg_dir(int64_t n, int parity, const double *part_rz,
                                                 const double *part_rr, double *scal, double tol2,
@@ -181,6 +187,7 @@ __global__ void __launch_bounds__(kVB) k_cg_dir(int64_t n, int parity, const dou
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     scal[1 - parity] = rz_new;
     scal[2] = rr;
+    scal[6] += 1.0;
     if (rr <= tol2 * scal[4]) scal[5] = 1.0;
   }
 }
@@ -270,9 +277,11 @@ static hipError_t newton_stats(ma_engine *e, const double *m, const double *nu, 
   return hipSuccess;
 }
 
-// Solve A x = b on the gauge-fixed Hessian (Morton order) with Jacobi PCG.
+// Solve H0 d = b on the gauge-fixed Hessian of the last evaluation (Morton
+// order) by the conjugate gradient, preconditioned by the aggregation
+// multigrid cycle (precond 0) or by Jacobi (precond 1).
 static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, int maxiter,
-                      int64_t *iters, double *rel_res) {
+                      int precond, int64_t *iters, double *rel_res) {
   SolverBuffers &S = e->sb;
   EvalBuffers &B = e->eb;
   const int64_t n = e->n;
@@ -291,12 +300,29 @@ static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, 
   MA_HIP_RET(S.red.ensure(4 * kVGrid + 16));
   double *part_a = S.red.p, *part_b = S.red.p + kVGrid, *part_c = S.red.p + 2 * kVGrid;
   double *scal = S.red.p + 4 * kVGrid;
-  hipLaunchKernelGGL(k_cg_init, dim3(kVGrid), dim3(kVB), 0, e->stream, n, b, B.diag.p, gi, x,
-                     S.r.p, S.z.p, S.p.p, part_a, part_b);
+  if (!e->amg) e->amg = amg_create();
+  {
+    const int levels = precond == 0 ? 0 : 1;  // Jacobi needs level 0 only
+    ma_status st = amg_setup(e, *e->amg, gi, levels);
+    if (st != MA_OK) return hipErrorUnknown;
+  }
+  const int32_t *ptr = B.csr_ptr.p, *col = B.csr_col.p;
+  const double *val = amg_level0_values(*e->amg), *dinv = amg_level0_dinv(*e->amg);
+  auto precondition = [&]() -> hipError_t {
+    if (precond == 0) return amg_apply(e, *e->amg, S.r.p, S.z.p, scal + 5);
+    hipLaunchKernelGGL(k_jacobi, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, n,
+                       dinv, S.r.p, S.z.p, scal);
+    return hipGetLastError();
+  };
+  hipLaunchKernelGGL(k_cg_begin, dim3(kVGrid), dim3(kVB), 0, e->stream, n, b, gi, x, S.r.p,
+                     part_b, scal);
+  MA_HIP_RET(precondition());
+  hipLaunchKernelGGL(k_dot_rz, dim3(kVGrid), dim3(kVB), 0, e->stream, n, S.r.p, S.z.p, S.p.p,
+                     part_a, scal);
   hipLaunchKernelGGL(k_cg_scal_init, dim3(1), dim3(kVB), 0, e->stream, part_a, part_b, scal);
   const double tol2 = rel_tol * rel_tol;
   int it = 0;
-  double sc[6];
+  double sc[7];
   MA_HIP_RET(hipMemcpyAsync(sc, scal, sizeof sc, hipMemcpyDeviceToHost, e->stream));
   MA_HIP_RET(hipStreamSynchronize(e->stream));
   if (sc[4] == 0.0) {
@@ -304,14 +330,17 @@ static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, 
     *rel_res = 0.0;
     return hipSuccess;
   }
-  const int batch = 32;
+  const int batch = precond == 0 ? 8 : 32;
   while (it < maxiter) {
     for (int q = 0; q < batch && it < maxiter; q++, it++) {
       const int parity = it & 1;
-      hipLaunchKernelGGL(k_spmv_dot, dim3(kVGrid), dim3(kVB), 0, e->stream, n, B.csr_ptr.p,
-                         B.csr_col.p, B.csr_val.p, S.p.p, S.Ap.p, gi, part_c);
+      hipLaunchKernelGGL(k_spmv_dot, dim3(kVGrid), dim3(kVB), 0, e->stream, n, ptr, col, val,
+                         S.p.p, S.Ap.p, part_c, scal);
       hipLaunchKernelGGL(k_cg_update, dim3(kVGrid), dim3(kVB), 0, e->stream, n, parity, part_c,
-                         scal, B.diag.p, gi, x, S.r.p, S.z.p, S.p.p, S.Ap.p, part_a, part_b);
+                         scal, x, S.r.p, S.p.p, S.Ap.p, part_b);
+      MA_HIP_RET(precondition());
+      hipLaunchKernelGGL(k_dot_rz, dim3(kVGrid), dim3(kVB), 0, e->stream, n, S.r.p, S.z.p,
+                         (double *)nullptr, part_a, scal);
       hipLaunchKernelGGL(k_cg_dir, dim3(kVGrid), dim3(kVB), 0, e->stream, n, parity, part_a,
                          part_b, scal, tol2, S.z.p, S.p.p);
     }
@@ -320,9 +349,10 @@ static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, 
     MA_HIP_RET(hipStreamSynchronize(e->stream));
     if (sc[5] != 0.0) break;
   }
-  *iters = it;
+  hipLaunchKernelGGL(k_set_zero_at, dim3(1), dim3(1), 0, e->stream, x, gi);
+  *iters = (int64_t)sc[6];
   *rel_res = std::sqrt(sc[2] / sc[4]);
-  return hipSuccess;
+  return hipGetLastError();
 }
 
 }  // namespace ma
@@ -389,6 +419,7 @@ void ma_default_solve_opts(ma_solve_opts *o) {
   o->cg_tol = 0.0;
   o->cg_maxiter = 20000;
   o->verbose = 0;
+  o->preconditioner = 0;
 }
 
 ma_status ma_create(int device, ma_engine **out) {
@@ -415,6 +446,7 @@ void ma_destroy(ma_engine *e) {
   hipStreamSynchronize(e->stream);
   for (auto &ev : e->ev) hipEventDestroy(ev);
   hipStreamDestroy(e->stream);
+  if (e->amg) amg_destroy(e->amg);
   e->tri.release();
   e->triv.release();
   e->vxy.release();
@@ -615,7 +647,7 @@ ma_status ma_solve_laplacian(ma_engine *e, const double *g, double *d, double re
   int64_t it = 0;
   double rr = 0.0;
   MA_HIP_TRY(pcg(e, S.g.p, S.d.p, rel_tol > 0 ? rel_tol : 1e-12,
-                 maxiter > 0 ? maxiter : 100000, &it, &rr));
+                 maxiter > 0 ? maxiter : 100000, 0, &it, &rr));
   MA_HIP_TRY(from_morton(e, S.d.p, d, false));
   if (res) *res = rr;
   if (iters) *iters = it;
@@ -696,7 +728,7 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
     }
     int64_t it = 0;
     double rr = 0.0;
-    MA_HIP_TRY(pcg(e, S.g.p, S.d.p, rel, o.cg_maxiter, &it, &rr));
+    MA_HIP_TRY(pcg(e, S.g.p, S.d.p, rel, o.cg_maxiter, o.preconditioner, &it, &rr));
     hipLaunchKernelGGL(k_neg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, n, S.d.p);
     st.cg_iter += it;
     MA_HIP_TRY(hipStreamSynchronize(e->stream));

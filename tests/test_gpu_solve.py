@@ -8,13 +8,14 @@ from mongeampere_amd import configs
 pytestmark = pytest.mark.gpu
 
 
-def test_solve_config1_maxrel(engine):
+@pytest.mark.parametrize("precond", ["amg", "jacobi"])
+def test_solve_config1_maxrel(engine, precond):
     """Config 1: damped Newton from w=0 to max |m-nu|/nu <= 1e-9; masses at
     the GPU solution checked with the oracle; weights equal the oracle's
     Newton solution modulo the gauge (the last weight is never moved)."""
     p = configs.config1()
     engine.set_mesh(p.mesh).set_diracs(p.X)
-    w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-9)
+    w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-9, preconditioner=precond)
     assert st.status == "MA_OK"
     assert st.max_rel_err <= 1e-9
     _, og, _ = oracle.kantorovich(p.mesh, p.X, w)
