@@ -1,14 +1,18 @@
-"""Benchmark: Kantorovich evaluations (masses + sparse Hessian) per second at
-N = 1M Diracs on one MI355X, plus the Newton solve time to max relative mass
-error 1e-8 in FP64 and mixed precision.
+"""Benchmark: Kantorovich evaluations (value, masses and the CSR Hessian on
+the device) per second at N = 1M Diracs on one MI355X, and the Newton solve
+time to max relative mass error 1e-8, mixed precision vs FP64.
 
 Workload (mixed precision at scale): unit square as a 1024 x 1024 vertex grid
 (2,093,058 triangles), P1 density 0.05 + 32 Gaussians (sigma in [0.01, 0.1]),
 N = 1,000,000 Diracs rejection-sampled from the density, lognormal(0, 0.5)
 target masses normalised to 1 (mongeampere_amd.configs.config3, seed 3).
 
-A step is one evaluation (value, masses and the CSR Hessian on the device) at
-device-resident weights.  Prints one JSON line.
+A step is one evaluation at device-resident weights w = 0 in the headline
+precision (mixed: FP32 cell geometry and quadrature, FP64 accumulation and
+Hessian).  `value` is evaluations/s over the K timed steps.  The line also
+carries the FP64 evaluation rate, the rates at the converged weights, both
+solve times, the roofline of the fragment kernel and the CPU baseline.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -33,13 +37,22 @@ def algorithmic_bytes(N, V, T, nnz, s_p=8, s_v=8, s_h=8):
     return N * (2 * s_p + 8 + 8) + 3 * V * s_v + 12 * T + nnz * (s_h + 4) + 4 * (N + 1)
 
 
+def frag_algorithmic_bytes(N, T, npairs, nvert, s=8):
+    """Bytes one launch of the fragment kernel must move at minimum: the
+    (cell, triangle) pair list (8 B per pair), every triangle record once
+    (6 coordinates + 3 density coefficients, 72 B), per Dirac its position,
+    vertex count and the mass / cost integrals (36 B), per cell vertex the
+    polygon read (x, y in s bytes each + a 4 B label) and the edge integral
+    (s bytes) + hit flag (4 B) written."""
+    return 8 * npairs + 72 * T + 36 * N + (3 * s + 8) * nvert
+
+
 def cpu_baseline_sample(p, frac=0.5):
     """Times the FP64 CPU oracle (one thread) on a window [0, frac]^2 of the
     same workload and scales the per-Dirac cost to the full problem."""
     import oracle
     from mongeampere_amd.configs import Mesh
 
-    n = p.grid_n
     keep = p.mesh.vxy.max(axis=1) <= frac + 1e-12
     vid = -np.ones(p.mesh.vxy.shape[0], np.int64)
     vid[keep] = np.arange(int(keep.sum()))
@@ -64,14 +77,39 @@ def cpu_baseline_sample(p, frac=0.5):
     }
 
 
+def time_evals(e, w_ptr, precision, steps):
+    """Times `steps` evaluations (host wall clock between synchronisations)
+    and collects the per-phase device times of each."""
+    import torch
+
+    frag, tot = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.kantorovich_device(w_ptr, precision)
+        info = e.eval_info()
+        frag.append(info["ms_frag"])
+        tot.append(info["ms_total"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return 1000.0 * wall / steps, float(np.mean(frag)), float(np.mean(tot)), info
+
+
+def log(msg):
+    """Progress on stderr (stdout carries only the JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the evaluation rate of the other precision")
     ap.add_argument("--N", type=int, default=1_000_000)
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--solve-precisions", default="fp64,mixed")
@@ -87,6 +125,7 @@ def main():
     from mongeampere_amd import configs
     from mongeampere_amd.api import Engine
 
+    log("generating the workload")
     p = configs.config3(N=args.N, n=args.grid)
     N = p.X.shape[0]
     e = Engine(0)
@@ -100,30 +139,25 @@ def main():
         del a, b
     torch.cuda.synchronize()
 
+    prec = args.precision
     for _ in range(args.warmup):
-        e.kantorovich_device(w_dev.data_ptr(), args.precision)
-    torch.cuda.synchronize()
-    cells_ms, tot_ms = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        e.kantorovich_device(w_dev.data_ptr(), args.precision)
-        info = e.eval_info()
-        cells_ms.append(info["ms_cells"])
-        tot_ms.append(info["ms_total"])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1000.0 * wall / args.steps
+        e.kantorovich_device(w_dev.data_ptr(), prec)
+    log(f"timing {args.steps} {prec} evaluations")
+    ms_per_step, k_ms, eval_ms, info = time_evals(e, w_dev.data_ptr(), prec, args.steps)
+    log(f"{ms_per_step:.2f} ms per evaluation")
     nnz = int(info["nnz"])
     V, T = p.mesh.vxy.shape[0], p.mesh.tri.shape[0]
-    B = algorithmic_bytes(N, V, T, nnz)
-    k_ms = float(np.mean(cells_ms))
-    achieved = B / (k_ms * 1e-3) / 1e9
+    s = 4 if prec == "mixed" else 8
+    B = algorithmic_bytes(N, V, T, nnz, s_p=s, s_v=s)
+    Bf = frag_algorithmic_bytes(N, T, int(info["npairs"]), int(info["nvert"]), s=s)
+    achieved = Bf / (k_ms * 1e-3) / 1e9
+    kname = "k_frag<0, float>" if prec == "mixed" else "k_frag<0, double>"
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "k_cells_C3_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "k_frag_C3_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             tj = json.load(f)
-        if tj.get("N") == N and tj.get("precision") == args.precision:
+        if tj.get("N") == N and tj.get("precision") == prec:
             traffic = tj.get("bytes_per_dispatch")
 
     out = {
@@ -137,31 +171,62 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f32+f64" if prec == "mixed" else "f64",
         "data": "synthetic (seeded generator, config 3)",
         "config": {"workload": f"C3: {args.grid}x{args.grid} P1 grid ({T} triangles), "
-                               f"N={N} Diracs, w=0, {args.precision}",
-                   "N": N, "triangles": T, "vertices": V, "nnz": nnz,
-                   "precision": args.precision},
-        "roofline": {"bound": "hbm", "kernel": "k_cells", "achieved": achieved,
+                               f"N={N} Diracs, w=0, {prec} evaluation",
+                   "N": N, "triangles": T, "vertices": V, "nnz": nnz, "precision": prec},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "algorithmic_bytes": B, "kernel_ms": k_ms},
-        "eval_ms_device": float(np.mean(tot_ms)),
-        "phases_ms": {k: info[k] for k in ("ms_weights", "ms_candidates", "ms_cells",
-                                           "ms_assemble")},
+                     "traffic": traffic, "algorithmic_bytes": Bf, "kernel_ms": k_ms},
+        "eval_roofline": {"algorithmic_bytes": B, "eval_ms_device": eval_ms,
+                          "achieved": B / (eval_ms * 1e-3) / 1e9,
+                          "frac": B / (eval_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "phases_ms": {k: info[k] for k in ("ms_weights", "ms_cell_build", "ms_candidates",
+                                           "ms_pairs", "ms_frag", "ms_cells", "ms_assemble")},
+        "counts": {k: int(info[k]) for k in ("npairs", "nfrag", "nvert", "nbinpath",
+                                             "ngeneral", "max_poly", "empty_cells",
+                                             "nredecide")},
     }
+    other = "fp64" if prec == "mixed" else "mixed"
+    if not args.no_compare:
+        e.kantorovich_device(w_dev.data_ptr(), other)
+        ms_o, _, _, _ = time_evals(e, w_dev.data_ptr(), other, max(3, args.steps // 2))
+        out[f"evals_per_s_{other}_w0"] = 1000.0 / ms_o
     if not args.no_solve:
-        for prec in [q for q in args.solve_precisions.split(",") if q]:
+        w_star = None
+        for q in [q for q in args.solve_precisions.split(",") if q]:
+            log(f"Newton solve, {q}")
             t0 = time.perf_counter()
-            w, st = e.ot_solve(p.nu, stop="maxrel", tol=1e-8, precision=prec,
-                               raise_on_error=False)
+            w, st = e.ot_solve(p.nu, stop="maxrel", tol=1e-8, precision=q, raise_on_error=False)
             dt = time.perf_counter() - t0
-            out[f"solve_{prec}"] = {"seconds": dt, "status": st.status, "newton": st.niter,
-                                    "evals": st.neval, "cg_iters": st.cg_iter,
-                                    "max_rel_err": st.max_rel_err,
-                                    "ms_eval": st.ms_eval, "ms_linear": st.ms_linear}
+            out[f"solve_{q}"] = {"seconds": dt, "status": st.status, "newton": st.niter,
+                                 "evals": st.neval, "evals_fp32": st.neval_fp32,
+                                 "cg_iters": st.cg_iter, "max_rel_err": st.max_rel_err,
+                                 "ms_eval": st.ms_eval, "ms_linear": st.ms_linear}
+            if st.status == "MA_OK":
+                w_star = w
+        if "solve_fp64" in out and "solve_mixed" in out:
+            out["solve_speedup_mixed_vs_fp64"] = (out["solve_fp64"]["seconds"]
+                                                  / out["solve_mixed"]["seconds"])
+        if w_star is not None:
+            # evaluation rates at the converged weights
+            wd = torch.tensor(w_star, dtype=torch.float64, device=dev)
+            for q in ("mixed", "fp64"):
+                e.kantorovich_device(wd.data_ptr(), q)
+                ms_q, _, _, _ = time_evals(e, wd.data_ptr(), q, max(3, args.steps // 2))
+                out[f"evals_per_s_{q}_wstar"] = 1000.0 / ms_q
     if not args.no_cpu:
+        log("CPU baseline sample")
         out["cpu_baseline"] = cpu_baseline_sample(p)
+        if "solve_mixed" in out:
+            # a CPU Newton solve needs at least as many evaluations as the
+            # GPU FP64 one: lower bound of its time from the CPU eval rate
+            ev = out["solve_fp64"]["evals"] if "solve_fp64" in out else out["solve_mixed"]["evals"]
+            cpu_solve_s = ev / out["cpu_baseline"]["value"]
+            out["cpu_solve_seconds_lower_bound"] = cpu_solve_s
+            out["solve_speedup_mixed_vs_cpu_lower_bound"] = (cpu_solve_s
+                                                             / out["solve_mixed"]["seconds"])
     e.close()
     print(json.dumps(out))
 

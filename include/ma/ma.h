@@ -72,12 +72,15 @@ typedef struct {
   double ms_total;      /* wall time of the solve                             */
   double ms_eval;       /* time in evaluations                                */
   double ms_linear;     /* time in linear solves                              */
+  int64_t neval_fp32;   /* evaluations with FP32 geometry (mixed precision)   */
 } ma_stats;
 
 /* Per-phase timing (ms, HIP events on the engine's stream) and counters of
  * the last evaluation. */
 typedef struct {
   double ms_total, ms_weights, ms_candidates, ms_cells, ms_assemble;
+                        /* ms_candidates: per-bin candidate lists, built only
+                           for cells that overflow the fast path            */
   double ms_cell_build, ms_pairs, ms_frag; /* parts of ms_cells             */
   int64_t nnz;          /* Hessian nonzeros (incl. diagonal)                  */
   int64_t nfrag;        /* non-empty (cell, triangle) fragments              */
@@ -86,8 +89,12 @@ typedef struct {
   int64_t empty_cells;  /* Diracs with zero mass                              */
   int64_t npairs;       /* (cell, triangle) pairs clipped                     */
   int64_t ngeneral;     /* cells evaluated by the general (overflow) kernel   */
-  int64_t nbinpath;     /* cells built from bin candidates (not certified by
-                           the site-centric search)                           */
+  int64_t nbinpath;     /* cells not certified by the ring search, finished
+                           by the max-weight pyramid walk                     */
+  int64_t nvert;        /* sum of fast-path cell polygon vertex counts        */
+  int64_t nredecide;    /* mixed mode: FP32 clip tests re-decided in FP64     */
+  int64_t nbincand;     /* cells rebuilt from per-bin candidate owners (walk
+                           over budget)                                       */
 } ma_eval_info;
 
 ma_status ma_create(int device, ma_engine **out);
@@ -131,19 +138,37 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
 /* Gauge-fixed Newton direction: solves H[:n-1,:n-1] d = g[:n-1], d[n-1] = 0
  * for the Hessian of the last evaluation.  Replaces solve_laplacian_matrix
  * (include/MA/optimal_transport.hpp:41-87).  *res gets ||H d - g|| over the
- * first n-1 rows. */
+ * first n-1 rows, relative to ||g||. */
 ma_status ma_solve_laplacian(ma_engine *e, const double *g, double *d, double rel_tol,
                              int maxiter, double *res, int64_t *iters);
+
+/* Same gauge-fixed solve for a caller-supplied symmetric CSR matrix (n x n,
+ * 0-based rowptr/col; e.g. a Hessian copied out earlier or assembled by the
+ * caller), the solve_laplacian_matrix(h, g) call shape of
+ * include/MA/optimal_transport.hpp:41-44.  Needs no mesh or Diracs.  The
+ * multigrid aggregates follow the index order, so a spatially coherent
+ * numbering converges fastest. */
+ma_status ma_solve_laplacian_csr(ma_engine *e, int64_t n, const int32_t *rowptr,
+                                 const int32_t *col, const double *val, const double *g,
+                                 double *d, double rel_tol, int maxiter, double *res,
+                                 int64_t *iters);
 
 /* Cell masses and first moments (int rho x, int rho y), replaces
  * MA::first_moment (include/MA/lloyd.hpp:30-69).  mom is n x 2 interleaved. */
 ma_status ma_first_moment(ma_engine *e, const double *w, double *mass, double *mom);
 
-/* Adjoint of the transport cost sum_i int_{Lag_i} rho |x - y_i|^2 at weights w
- * (normally the converged ones): dY (n x 2 interleaved) = d/dy_i with w held
- * fixed (at optimal weights this equals the total derivative,
- * 2 m_i (y_i - b_i)); dval (nv) = int (|x - y_i|^2 - w_i) phi_v(x) dx, the
- * derivative w.r.t. vertex density values with w held fixed. */
+/* Adjoint of the transport cost.  With the dual value
+ *   Phi(Y, w, rho) = int rho(x) min_i (|x - y_i|^2 - w_i) dx,
+ * W2^2(rho, sum_i nu_i delta_{y_i}) = max_w Phi + <nu, w>, so at the optimal
+ * weights (pass the converged w) the envelope theorem gives
+ *   dY   (n x 2 interleaved, may be NULL)  dW2^2/dy_i = 2 m_i (y_i - b_i)
+ *        (b_i the density centroid of cell i: the Lloyd form of
+ *        include/MA/lloyd.hpp:126-144),
+ *   dval (nv, may be NULL)  dW2^2/drho_v = sum_i int_{Lag_i} (|x - y_i|^2 - w_i)
+ *        phi_v(x) dx, phi_v the P1 hat function of mesh vertex v.
+ * At other weights these are the partial derivatives of Phi at fixed w.  The
+ * diagram is recomputed (one evaluation); nothing is kept from earlier calls.
+ * No counterpart in the reference (it has no derivatives w.r.t. Y or rho). */
 ma_status ma_adjoint(ma_engine *e, const double *w, double *dY, double *dval);
 
 ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info);
@@ -151,8 +176,9 @@ ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info);
 /* Engine options.  MA_OPT_FORCE_GENERAL (0/1): evaluate every cell with the
  * general kernel instead of the register/LDS fast path (for testing the
  * overflow path); MA_OPT_CAP_CAND: initial capacity of the per-bin candidate
- * lists (grows automatically on overflow). */
-typedef enum { MA_OPT_FORCE_GENERAL = 1, MA_OPT_CAP_CAND = 2 } ma_option;
+ * lists (grows automatically on overflow); MA_OPT_GRAPHS (0/1, default 1):
+ * replay batches of conjugate-gradient iterations from a HIP graph. */
+typedef enum { MA_OPT_FORCE_GENERAL = 1, MA_OPT_CAP_CAND = 2, MA_OPT_GRAPHS = 3 } ma_option;
 ma_status ma_set_option(ma_engine *e, int option, int64_t value);
 void ma_default_solve_opts(ma_solve_opts *o);
 

@@ -33,7 +33,7 @@ class SolveOpts(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("niter", I64), ("neval", I64), ("cg_iter", I64), ("bad_index", I64),
                 ("max_rel_err", F64), ("l2_err", F64), ("fval", F64), ("ms_total", F64),
-                ("ms_eval", F64), ("ms_linear", F64)]
+                ("ms_eval", F64), ("ms_linear", F64), ("neval_fp32", I64)]
 
 
 class EvalInfo(ctypes.Structure):
@@ -41,7 +41,8 @@ class EvalInfo(ctypes.Structure):
                 ("ms_cells", F64), ("ms_assemble", F64), ("ms_cell_build", F64),
                 ("ms_pairs", F64), ("ms_frag", F64), ("nnz", I64), ("nfrag", I64),
                 ("ncand", I64), ("max_poly", I64), ("empty_cells", I64), ("npairs", I64),
-                ("ngeneral", I64), ("nbinpath", I64)]
+                ("ngeneral", I64), ("nbinpath", I64), ("nvert", I64),
+                ("nredecide", I64), ("nbincand", I64)]
 
 
 # (name, restype, argtypes) of every exported entry point of include/ma/ma.h
@@ -56,6 +57,7 @@ SIGNATURES = [
     ("ma_hessian_copy", I32, [P, P, P, P]),
     ("ma_ot_solve", I32, [P, P, P, I32, P, P]),
     ("ma_solve_laplacian", I32, [P, P, P, F64, I32, P, P]),
+    ("ma_solve_laplacian_csr", I32, [P, I64, P, P, P, P, P, F64, I32, P, P]),
     ("ma_first_moment", I32, [P, P, P, P]),
     ("ma_adjoint", I32, [P, P, P, P]),
     ("ma_get_eval_info", I32, [P, P]),

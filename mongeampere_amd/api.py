@@ -47,6 +47,7 @@ class Statistics:
     ms_linear: float = 0.0
     status: str = "MA_OK"
     bad_index: int = -1
+    neval_fp32: int = 0
 
 
 class Engine:
@@ -86,7 +87,7 @@ class Engine:
             raise EngineError(st, msg.decode() if msg else "")
 
     def set_option(self, name, value):
-        key = {"force_general": 1, "cap_cand": 2}[name]
+        key = {"force_general": 1, "cap_cand": 2, "graphs": 3}[name]
         self._check(self._L.ma_set_option(self._h, key, int(value)))
         return self
 
@@ -158,6 +159,25 @@ class Engine:
                                                ctypes.byref(res), ctypes.byref(it)))
         return d, res.value, it.value
 
+    def solve_laplacian_matrix(self, h, g, rel_tol=1e-12, maxiter=0):
+        """Gauge-fixed solve for a caller-supplied sparse matrix h
+        (``solve_laplacian_matrix(h, g)``, include/MA/optimal_transport.hpp:41-87):
+        d[:-1] solves h[:-1, :-1] d = g[:-1], d[-1] = 0."""
+        h = sp.csr_matrix(h)
+        h.sort_indices()
+        n = h.shape[0]
+        ptr = np.ascontiguousarray(h.indptr, np.int32)
+        col = np.ascontiguousarray(h.indices, np.int32)
+        val = np.ascontiguousarray(h.data, np.float64)
+        g = np.ascontiguousarray(g, np.float64)
+        d = np.empty(n)
+        res = ctypes.c_double()
+        it = ctypes.c_int64()
+        self._check(self._L.ma_solve_laplacian_csr(self._h, n, _ptr(ptr), _ptr(col), _ptr(val),
+                                                   _ptr(g), _ptr(d), rel_tol, int(maxiter),
+                                                   ctypes.byref(res), ctypes.byref(it)))
+        return d, res.value, it.value
+
     def ot_solve(self, nu, x=None, eps_g=1e-7, maxiter=100, verbose=False, stop="l2",
                  tol=1e-8, precision="fp64", cg_tol=0.0, cg_maxiter=20000,
                  alpha_min=2.0 ** -30, raise_on_error=True, preconditioner="amg"):
@@ -181,7 +201,7 @@ class Engine:
                                  ctypes.byref(s))
         stats = Statistics(s.niter, s.neval, s.cg_iter, s.max_rel_err, s.l2_err, s.fval,
                            s.ms_total, s.ms_eval, s.ms_linear, _lib.STATUS.get(st, str(st)),
-                           s.bad_index)
+                           s.bad_index, s.neval_fp32)
         if st != 0 and raise_on_error and st not in (_lib.MA_NOT_CONVERGED,):
             self._check(st)
         return w, stats

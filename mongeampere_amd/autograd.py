@@ -1,0 +1,59 @@
+"""``torch.autograd.Function`` around the engine's optimal transport solve and
+its adjoint, for tests and gradient-driven callers.
+
+``W2Squared.apply(Y, vval, engine, mesh, nu, solve_kw)`` returns the squared
+Wasserstein distance W2^2 between the P1 density with vertex values ``vval``
+on ``mesh`` and the discrete measure sum_i nu_i delta_{Y_i}.  Forward solves
+the semi-discrete problem on the GPU (damped Newton, include/MA/
+optimal_transport.hpp:89-193); by Kantorovich duality
+
+    W2^2 = max_w Phi(Y, w, rho) + <nu, w>,
+    Phi(Y, w, rho) = int rho(x) min_i (|x - y_i|^2 - w_i) dx = -fval(w),
+
+with fval the value returned by include/MA/kantorovich.hpp:35-141.  Backward
+recomputes the diagram at the converged weights (ma_adjoint: one evaluation,
+nothing of the forward diagram is stored) and applies the envelope theorem:
+dW2^2/dy_i = 2 m_i (y_i - b_i) and dW2^2/drho_v = int phi_v min_i(...) dx.
+
+The density gradient is meaningful along perturbations that keep the total
+mass equal to sum(nu) (otherwise the problem has no solution); callers
+differentiating w.r.t. ``vval`` should use mass-preserving directions.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .configs import Mesh
+
+
+class W2Squared(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, Y, vval, engine, mesh, nu, solve_kw=None):
+        kw = {"stop": "maxrel", "tol": 1e-10, "maxiter": 100}
+        kw.update(solve_kw or {})
+        Yn = Y.detach().cpu().numpy().astype(np.float64)
+        vn = vval.detach().cpu().numpy().astype(np.float64)
+        m = Mesh(mesh.vxy, vn, mesh.tri)
+        engine.set_mesh(m).set_diracs(Yn)
+        w, st = engine.ot_solve(np.asarray(nu, np.float64), **kw)
+        if st.status != "MA_OK":
+            raise RuntimeError(f"optimal transport solve failed: {st.status}")
+        ctx.engine = engine
+        ctx.w = w
+        ctx.save_for_backward(Y, vval)
+        return torch.tensor(-st.fval, dtype=torch.float64, device=Y.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        Y, vval = ctx.saved_tensors
+        dY, dval = ctx.engine.adjoint(ctx.w)  # diagram recomputed at the optimum
+        g = grad_out.detach().cpu().double()
+        gY = torch.from_numpy(dY).to(Y.device, Y.dtype) * g.to(Y.device)
+        gv = torch.from_numpy(dval).to(vval.device, vval.dtype) * g.to(vval.device)
+        return gY, gv, None, None, None, None
+
+
+def w2_squared(Y, vval, engine, mesh, nu, **solve_kw):
+    """Functional form of :class:`W2Squared`."""
+    return W2Squared.apply(Y, vval, engine, mesh, nu, solve_kw)

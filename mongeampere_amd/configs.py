@@ -183,8 +183,74 @@ def config4(N=200_000, n=512):
     return Problem("C4", mesh, X, nu, n)
 
 
+def annulus_mesh(nv, r0=0.2, r1=1.0, rng=None):
+    """Unstructured annulus r0 <= r <= r1: a blue-noise vertex set (a lattice
+    of spacing h with uniform jitter of 0.3 h, so any two vertices are at
+    least 0.4 h apart) inside the annulus plus vertices every h on both
+    boundary circles, Delaunay-triangulated (scipy / Qhull); triangles whose
+    centroid lies in the hole are dropped, so the domain is non-convex."""
+    from scipy.spatial import Delaunay
+
+    rng = rng or np.random.default_rng(0)
+    area = np.pi * (r1 * r1 - r0 * r0)
+    h = np.sqrt(area / nv)
+    g = np.arange(-r1, r1 + h, h)
+    P = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
+    P = P + rng.uniform(-0.3 * h, 0.3 * h, size=P.shape)
+    r = np.hypot(P[:, 0], P[:, 1])
+    P = P[(r > r0 + 0.5 * h) & (r < r1 - 0.5 * h)]
+    circ = []
+    for rad in (r0, r1):
+        m = int(np.ceil(2 * np.pi * rad / h))
+        t = 2 * np.pi * np.arange(m) / m
+        circ.append(np.stack([rad * np.cos(t), rad * np.sin(t)], 1))
+    P = np.concatenate([P] + circ)
+    tri = Delaunay(P).simplices.astype(np.int32)
+    c = P[tri].mean(axis=1)
+    tri = tri[np.hypot(c[:, 0], c[:, 1]) > r0]
+    # counter-clockwise
+    a = P[tri]
+    cw = ((a[:, 1, 0] - a[:, 0, 0]) * (a[:, 2, 1] - a[:, 0, 1])
+          - (a[:, 2, 0] - a[:, 0, 0]) * (a[:, 1, 1] - a[:, 0, 1])) < 0
+    tri[cw] = tri[cw][:, [0, 2, 1]]
+    return P, tri
+
+
+def config5(N=4_000_000, nv=500_000):
+    """Robustness plus scale: annulus r in [0.2, 1] with nv unstructured
+    vertices (annulus_mesh), vertex values lognormal(0, 1); N Diracs, half on
+    a square lattice restricted to the annulus with jitter 1e-7 (near-
+    cocircular ties), half uniform in the annulus; nu proportional to 1."""
+    rng = np.random.Generator(np.random.PCG64(5))
+    vxy, tri = annulus_mesh(nv, rng=rng)
+    vval = rng.lognormal(0.0, 1.0, size=vxy.shape[0])
+    mesh = _normalise(Mesh(vxy, vval, tri))
+    r0, r1 = 0.2, 1.0
+    nl = N // 2
+    s = np.sqrt(np.pi * (r1 * r1 - r0 * r0) / nl)
+    g = (np.arange(-r1, r1, s) + 0.5 * s)
+    L = np.stack(np.meshgrid(g, g, indexing="ij"), -1).reshape(-1, 2)
+    rr = np.hypot(L[:, 0], L[:, 1])
+    # strictly inside the meshed annulus (its boundary is a polygon)
+    L = L[(rr > r0 + 2 * s) & (rr < r1 - 2 * s)]
+    L = L + rng.uniform(-1e-7, 1e-7, size=L.shape)
+    nu_ = N - L.shape[0]
+    U = []
+    have = 0
+    while have < nu_:
+        m = 2 * (nu_ - have) + 1024
+        p = rng.uniform(-r1, r1, size=(m, 2))
+        rp = np.hypot(p[:, 0], p[:, 1])
+        p = p[(rp > r0 + 2 * s) & (rp < r1 - 2 * s)]
+        U.append(p)
+        have += p.shape[0]
+    X = np.concatenate([L, np.concatenate(U)[:nu_]])
+    nu = np.full(X.shape[0], 1.0 / X.shape[0])
+    return Problem("C5", mesh, X, nu, 0)
+
+
 def config(k, **kw):
-    return {1: config1, 2: config2, 3: config3, 4: config4}[k](**kw)
+    return {1: config1, 2: config2, 3: config3, 4: config4, 5: config5}[k](**kw)
 
 
 def image_to_pl(image):

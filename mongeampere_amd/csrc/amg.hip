@@ -197,13 +197,7 @@ struct Amg {
   std::vector<AmgLevel> lv;
   DBuf<double> dense;
   int nlev = 0;
-  ~Amg() {
-    for (auto &l : lv) {
-      l.ptr.release(); l.col.release(); l.val.release(); l.dinv.release();
-      l.x.release(); l.y.release(); l.b.release();
-    }
-    dense.release();
-  }
+  const int32_t *ptr0 = nullptr, *col0 = nullptr;  // level-0 pattern (the caller's CSR)
 };
 
 static inline unsigned nb2(int64_t n) { return (unsigned)((n + kVB2 - 1) / kVB2); }
@@ -213,35 +207,39 @@ void amg_destroy(Amg *a) { delete a; }
 const double *amg_level0_values(const Amg &a) { return a.lv[0].val.p; }
 const double *amg_level0_dinv(const Amg &a) { return a.lv[0].dinv.p; }
 
-// Builds the hierarchy from the Morton-order CSR of the last evaluation
-// (max_levels = 1: the gauge-fixed level-0 matrix only).
-ma_status amg_setup(ma_engine *e, Amg &A, int32_t gi, int max_levels) {
-  const int64_t n = e->n;
-  EvalBuffers &B = e->eb;
-  A.lv.clear();
-  A.lv.reserve(24);
-  A.lv.emplace_back();
-  AmgLevel &L0 = A.lv.back();
+// Builds the hierarchy from the CSR matrix M (max_levels = 1: the
+// gauge-fixed level-0 matrix only).  Aggregation follows the index order, so
+// a spatially coherent order (Morton) gives compact aggregates.
+ma_status amg_setup(ma_engine *e, Amg &A, const CsrView &M, int32_t gi, int max_levels) {
+  const int64_t n = M.n;
+  A.ptr0 = M.ptr;
+  A.col0 = M.col;
+  // levels are kept between setups (buffers only grow); nlev counts the
+  // ones in use
+  A.lv.reserve(32);
+  if (A.lv.empty()) A.lv.emplace_back();
+  size_t used = 1;
+  AmgLevel &L0 = A.lv[0];
   L0.n = n;
-  MA_HIP_TRY(L0.val.ensure((size_t)e->nnz + 1));
+  MA_HIP_TRY(L0.val.ensure((size_t)M.nnz + 1));
   MA_HIP_TRY(L0.dinv.ensure((size_t)n));
   MA_HIP_TRY(L0.x.ensure((size_t)n));
   MA_HIP_TRY(L0.y.ensure((size_t)n));
-  hipLaunchKernelGGL(k_amg_level0, dim3(nb2(n)), dim3(kVB2), 0, e->stream, n, B.csr_ptr.p,
-                     B.csr_col.p, B.csr_val.p, gi, L0.val.p, L0.dinv.p);
+  hipLaunchKernelGGL(k_amg_level0, dim3(nb2(n)), dim3(kVB2), 0, e->stream, n, M.ptr, M.col,
+                     M.val, gi, L0.val.p, L0.dinv.p);
   if (max_levels == 1) {
     A.nlev = 1;
     MA_HIP_TRY(hipGetLastError());
     return MA_OK;
   }
-  while (A.lv.back().n > kCoarsest) {
-    const size_t l = A.lv.size() - 1;
+  while (A.lv[used - 1].n > kCoarsest && used < 32) {
+    const size_t l = used - 1;
     const int64_t nf = A.lv[l].n, nc = (nf + kAgg - 1) / kAgg;
-    const int32_t *fptr = (l == 0) ? B.csr_ptr.p : A.lv[l].ptr.p;
-    const int32_t *fcol = (l == 0) ? B.csr_col.p : A.lv[l].col.p;
+    const int32_t *fptr = (l == 0) ? M.ptr : A.lv[l].ptr.p;
+    const int32_t *fcol = (l == 0) ? M.col : A.lv[l].col.p;
     const double *fval = A.lv[l].val.p;
-    A.lv.emplace_back();
-    AmgLevel &C = A.lv.back();
+    if (A.lv.size() <= used) A.lv.emplace_back();
+    AmgLevel &C = A.lv[used++];
     C.n = nc;
     MA_HIP_TRY(C.ptr.ensure((size_t)nc + 1));
     MA_HIP_TRY(C.dinv.ensure((size_t)nc));
@@ -263,23 +261,22 @@ ma_status amg_setup(ma_engine *e, Amg &A, int32_t gi, int max_levels) {
     hipLaunchKernelGGL(k_amg_coarse<true>, dim3(nb2(nc)), dim3(kVB2), 0, e->stream, nc, nf,
                        fptr, fcol, fval, C.ptr.p, C.col.p, C.val.p, C.dinv.p);
   }
-  const AmgLevel &Lc = A.lv.back();
-  const int32_t *cptr = (A.lv.size() == 1) ? B.csr_ptr.p : Lc.ptr.p;
-  const int32_t *ccol = (A.lv.size() == 1) ? B.csr_col.p : Lc.col.p;
+  const AmgLevel &Lc = A.lv[used - 1];
+  const int32_t *cptr = (used == 1) ? M.ptr : Lc.ptr.p;
+  const int32_t *ccol = (used == 1) ? M.col : Lc.col.p;
   MA_HIP_TRY(A.dense.ensure((size_t)(Lc.n * Lc.n)));
   hipLaunchKernelGGL(k_amg_dense_chol, dim3(1), dim3(256), 0, e->stream, Lc.n, cptr, ccol,
                      Lc.val.p, A.dense.p);
-  A.nlev = (int)A.lv.size();
+  A.nlev = (int)used;
   MA_HIP_TRY(hipGetLastError());
   return MA_OK;
 }
 
 // z = M^-1 r (one V-cycle); done: device flag, cycles are skipped once set.
 hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const double *done) {
-  EvalBuffers &B = e->eb;
   const int nl = A.nlev;
-  auto ptr_of = [&](int l) { return l == 0 ? B.csr_ptr.p : A.lv[l].ptr.p; };
-  auto col_of = [&](int l) { return l == 0 ? B.csr_col.p : A.lv[l].col.p; };
+  auto ptr_of = [&](int l) { return l == 0 ? A.ptr0 : A.lv[l].ptr.p; };
+  auto col_of = [&](int l) { return l == 0 ? A.col0 : A.lv[l].col.p; };
   // downward
   for (int l = 0; l < nl - 1; l++) {
     AmgLevel &L = A.lv[l];

@@ -1,6 +1,8 @@
 // Register/LDS-resident evaluation kernels:
-//   k_cell_build  one thread per Dirac: power cell clipped in place in LDS
-//                 (FP64, at most kMaxFast vertices), written once to HBM;
+//   k_cell_local  one thread per Dirac: power cell clipped in place in LDS
+//                 (at most kMaxFast vertices) by the Diracs of the
+//                 surrounding bins, certified ring by ring or through the
+//                 max-weight pyramid, written once to HBM;
 //   k_pairs       (cell, triangle) pairs whose bounding boxes meet;
 //   k_frag        one thread per pair: the cell polygon streamed through a
 //                 three-stage Sutherland-Hodgman pipeline (one stage per
@@ -8,7 +10,14 @@
 //                 quadrature of the fragment;
 //   k_rows        one thread per Dirac: Hessian row from the edge integrals.
 // Cells that overflow kMaxFast vertices are flagged and evaluated by the
-// general kernel (k_cells_general).
+// general FP64 kernel (k_cells_general).
+//
+// The geometry kernels are templated on the arithmetic type R.  R = double is
+// the FP64 mode.  R = float is the FP32 part of the mixed mode: coordinates
+// local to the Dirac are formed in FP64 and rounded once to FP32; clip
+// decisions whose FP32 value lies within the FP32 error bound of zero are
+// re-decided in FP64 (and counted); fragment masses and cost integrals are
+// accumulated with Kahan compensation and added to FP64 totals.
 //
 // Reference behaviour (paths relative to the MongeAmpere++ tree):
 //   strict tie rule, edge labels   include/MA/voronoi_triangulation_intersection.hpp:125-198
@@ -23,6 +32,25 @@ namespace ma {
 constexpr int kMaxFast = 16;
 constexpr int kCellBlock = 128;
 constexpr int kLocalRings = 3;  // site-centric search radius, in bins
+constexpr int kWalkBudget = 192;  // pyramid nodes a cell may visit before the bin path
+
+// Tolerances per arithmetic type (all relative to the local scale):
+//   snap  side of a triangle edge treated as zero (vertex on the edge line);
+//   area  fragments below this area are degenerate (zero in exact arithmetic);
+//   redo  cell-clip values below this are re-decided in FP64 (FP32 only);
+//   pad   relative enlargement of the polygon radius in the pruning test,
+//         covering the rounding of the stored vertices.
+template <class R>
+struct Tol;
+template <>
+struct Tol<double> {
+  static constexpr double snap = 1e-13, area = 1e-13, redo = 0.0, pad = 0.0;
+};
+template <>
+struct Tol<float> {
+  static constexpr float snap = 1e-6f, area = 4e-6f, redo = 1e-6f;
+  static constexpr double pad = 1e-5;
+};
 
 struct CellArgs {
   Grid g;
@@ -33,32 +61,93 @@ struct CellArgs {
   const int32_t *tbin_start, *tbin_items, *bin_start;
   double dom[4];     // domain bounding box
   const TriData *tri;
-  double *px, *py;   // cell vertices (local to the Dirac), [v * n + k]
-  int32_t *plab;     // neighbour across edge v -> v+1 (-1: bin-rectangle edge)
+  double *px, *py;   // FP64 cell vertices (local to the Dirac), [v * n + k]
+  float *pxf, *pyf;  // FP32 cell vertices (mixed mode)
+  int32_t *plab;     // neighbour across edge v -> v+1 (-1: domain edge)
   int32_t *pn;       // vertex count, -1 = handled by the general kernel
   double *bbox;      // absolute bounding box of the cell (4 per Dirac)
   int32_t *tcount;   // candidate triangles per cell
   double *mass, *intg, *mom;  // mom: local first moments, interleaved (x, y)
   double *eacc;      // edge integrals int rho ds, [v * n + k]
+  float *eaccf;      // FP32 edge integrals (mixed mode)
   int32_t *ehit;     // edge seen in a fragment
+  const int32_t *triv;  // triangle vertex ids (counter-clockwise, as in tri)
+  double *dval;      // adjoint w.r.t. vertex density values (mode 2)
   int32_t *flags;
+  unsigned long long *counters;
   int force_general;  // route every cell to the general kernel (testing)
 };
 
-// In-place clip of the LDS polygon of thread t by {X : kk - 2 X.Y > 0}
-// (strict: points with pi_k == pi_j are outside).  The surviving vertices
-// form one cyclic run; it is moved to the front, followed by the exit point
-// X (label = new neighbour) and the entry point E (label of the edge it lies
-// on).  Returns the new vertex count; sets fail on overflow or when rounding
-// breaks the single-run structure.
-__device__ inline int clip_lds(double (*sx)[kCellBlock], double (*sy)[kCellBlock],
-                               int (*sl)[kCellBlock], int t, int n, double Yx, double Yy,
-                               double kk, int label, bool &fail) {
-  uint32_t in = 0u;
-  for (int v = 0; v < n; v++) {
-    const double s = kk - 2.0 * (sx[v][t] * Yx + sy[v][t] * Yy);
-    if (s > 0.0) in |= 1u << v;
+template <class R>
+struct CellBuf;
+template <>
+struct CellBuf<double> {
+  __device__ static double *px(const CellArgs &a) { return a.px; }
+  __device__ static double *py(const CellArgs &a) { return a.py; }
+  __device__ static double *eacc(const CellArgs &a) { return a.eacc; }
+};
+template <>
+struct CellBuf<float> {
+  __device__ static float *px(const CellArgs &a) { return a.pxf; }
+  __device__ static float *py(const CellArgs &a) { return a.pyf; }
+  __device__ static float *eacc(const CellArgs &a) { return a.eaccf; }
+};
+
+// The LDS polygon of one thread.
+template <class R>
+struct CellLDS {
+  R (*sx)[kCellBlock];
+  R (*sy)[kCellBlock];
+  int (*sl)[kCellBlock];
+  int t;
+  __device__ double rho2(int n) const {
+    R r = R(0);
+    for (int v = 0; v < n; v++) r = fmax(r, sx[v][t] * sx[v][t] + sy[v][t] * sy[v][t]);
+    return (double)r * (1.0 + Tol<R>::pad);
   }
+  __device__ void rect(R X0, R Y0, R X1, R Y1) {
+    sx[0][t] = X0; sy[0][t] = Y0; sl[0][t] = -1;
+    sx[1][t] = X1; sy[1][t] = Y0; sl[1][t] = -1;
+    sx[2][t] = X1; sy[2][t] = Y1; sl[2][t] = -1;
+    sx[3][t] = X0; sy[3][t] = Y1; sl[3][t] = -1;
+  }
+};
+
+// Half-plane of competitor j seen from Dirac k, in coordinates local to y_k:
+// X stays in the cell of k iff kk - 2 X.Y > 0 (strict: ties are outside),
+// Y = y_j - y_k, kk = |Y|^2 + w_k - w_j.  Both precisions are carried.
+template <class R>
+struct HalfPlane {
+  R Yx, Yy, kk;
+  double Yx64, Yy64, kk64;
+  int label;
+  // value at (x, y); an FP32 value within the FP32 error bound of zero is
+  // replaced by the FP64 value at the same (stored) point
+  __device__ R value(R x, R y, int *nredo) const {
+    const R s = kk - R(2) * (x * Yx + y * Yy);
+    if (Tol<R>::redo > R(0)) {
+      const R scale = fabs(kk) + R(2) * (fabs(x * Yx) + fabs(y * Yy));
+      if (fabs(s) <= Tol<R>::redo * scale) {
+        if (nredo) (*nredo)++;
+        return (R)(kk64 - 2.0 * ((double)x * Yx64 + (double)y * Yy64));
+      }
+    }
+    return s;
+  }
+};
+
+// In-place clip of the LDS polygon of thread t by a half-plane.  The
+// surviving vertices form one cyclic run; it is moved to the front, followed
+// by the exit point X (label = new neighbour) and the entry point E (label of
+// the edge it lies on).  Returns the new vertex count; sets fail on overflow
+// or when rounding breaks the single-run structure.
+template <class R>
+__device__ inline int clip_lds(CellLDS<R> &L, int n, const HalfPlane<R> &H, bool &fail,
+                               int &nredo) {
+  const int t = L.t;
+  uint32_t in = 0u;
+  for (int v = 0; v < n; v++)
+    if (H.value(L.sx[v][t], L.sy[v][t], &nredo) > R(0)) in |= 1u << v;
   const uint32_t full = (1u << n) - 1u;
   if (in == full) return n;
   if (in == 0u) return 0;
@@ -76,23 +165,23 @@ __device__ inline int clip_lds(double (*sx)[kCellBlock], double (*sy)[kCellBlock
     fail = true;
     return n;
   }
-  double ax = sx[xk][t], ay = sy[xk][t], bx = sx[xk1][t], by = sy[xk1][t];
-  double sa = kk - 2.0 * (ax * Yx + ay * Yy), sb = kk - 2.0 * (bx * Yx + by * Yy);
-  double tt = sa / (sa - sb);
-  const double Xx = ax + tt * (bx - ax), Xy = ay + tt * (by - ay);
-  ax = sx[ek][t]; ay = sy[ek][t]; bx = sx[ek1][t]; by = sy[ek1][t];
-  sa = kk - 2.0 * (ax * Yx + ay * Yy);
-  sb = kk - 2.0 * (bx * Yx + by * Yy);
+  R ax = L.sx[xk][t], ay = L.sy[xk][t], bx = L.sx[xk1][t], by = L.sy[xk1][t];
+  R sa = H.value(ax, ay, nullptr), sb = H.value(bx, by, nullptr);
+  R tt = sa / (sa - sb);
+  const R Xx = ax + tt * (bx - ax), Xy = ay + tt * (by - ay);
+  ax = L.sx[ek][t]; ay = L.sy[ek][t]; bx = L.sx[ek1][t]; by = L.sy[ek1][t];
+  sa = H.value(ax, ay, nullptr);
+  sb = H.value(bx, by, nullptr);
   tt = sa / (sa - sb);
-  const double Ex = ax + tt * (bx - ax), Ey = ay + tt * (by - ay);
-  const int El = sl[ek][t];
+  const R Ex = ax + tt * (bx - ax), Ey = ay + tt * (by - ay);
+  const int El = L.sl[ek][t];
   const int s0 = ek1;
   int px, pe;
   if (s0 <= xk) {
     for (int i = 0; i < cnt_in; i++) {
-      sx[i][t] = sx[s0 + i][t];
-      sy[i][t] = sy[s0 + i][t];
-      sl[i][t] = sl[s0 + i][t];
+      L.sx[i][t] = L.sx[s0 + i][t];
+      L.sy[i][t] = L.sy[s0 + i][t];
+      L.sl[i][t] = L.sl[s0 + i][t];
     }
     px = cnt_in;
     pe = cnt_in + 1;
@@ -100,53 +189,36 @@ __device__ inline int clip_lds(double (*sx)[kCellBlock], double (*sy)[kCellBlock
     const int tl = n - s0, dest = xk + 3;
     if (dest < s0) {
       for (int i = 0; i < tl; i++) {
-        sx[dest + i][t] = sx[s0 + i][t];
-        sy[dest + i][t] = sy[s0 + i][t];
-        sl[dest + i][t] = sl[s0 + i][t];
+        L.sx[dest + i][t] = L.sx[s0 + i][t];
+        L.sy[dest + i][t] = L.sy[s0 + i][t];
+        L.sl[dest + i][t] = L.sl[s0 + i][t];
       }
     } else if (dest > s0) {
       for (int i = tl - 1; i >= 0; i--) {
-        sx[dest + i][t] = sx[s0 + i][t];
-        sy[dest + i][t] = sy[s0 + i][t];
-        sl[dest + i][t] = sl[s0 + i][t];
+        L.sx[dest + i][t] = L.sx[s0 + i][t];
+        L.sy[dest + i][t] = L.sy[s0 + i][t];
+        L.sl[dest + i][t] = L.sl[s0 + i][t];
       }
     }
     px = xk + 1;
     pe = xk + 2;
   }
-  sx[px][t] = Xx;
-  sy[px][t] = Xy;
-  sl[px][t] = label;
-  sx[pe][t] = Ex;
-  sy[pe][t] = Ey;
-  sl[pe][t] = El;
+  L.sx[px][t] = Xx;
+  L.sy[px][t] = Xy;
+  L.sl[px][t] = H.label;
+  L.sx[pe][t] = Ex;
+  L.sy[pe][t] = Ey;
+  L.sl[pe][t] = El;
   return m;
 }
-
-// The LDS polygon of one thread.
-struct CellLDS {
-  double (*sx)[kCellBlock];
-  double (*sy)[kCellBlock];
-  int (*sl)[kCellBlock];
-  int t;
-  __device__ double rho2(int n) const {
-    double r = 0.0;
-    for (int v = 0; v < n; v++) r = fmax(r, sx[v][t] * sx[v][t] + sy[v][t] * sy[v][t]);
-    return r;
-  }
-  __device__ void rect(double X0, double Y0, double X1, double Y1) {
-    sx[0][t] = X0; sy[0][t] = Y0; sl[0][t] = -1;
-    sx[1][t] = X1; sy[1][t] = Y0; sl[1][t] = -1;
-    sx[2][t] = X1; sy[2][t] = Y1; sl[2][t] = -1;
-    sx[3][t] = X0; sy[3][t] = Y1; sl[3][t] = -1;
-  }
-};
 
 // Clips the cell of Dirac k by the power half-plane of candidate j.  A cheap
 // distance test first discards candidates that cannot reach the polygon:
 // pi_j - pi_k >= d^2 - 2 rho d + w_k - w_j on the disc of radius rho.
+template <class R>
 __device__ inline void consider(const CellArgs &a, int64_t k, int32_t j, double yx, double yy,
-                                double wk, CellLDS &L, int &n, double &rho2, bool &fail) {
+                                double wk, CellLDS<R> &L, int &n, double &rho2, bool &fail,
+                                int &nredo) {
   if (j == k || n == 0) return;
   const double Yx = a.pos[2 * j] - yx, Yy = a.pos[2 * j + 1] - yy, wj = a.w[j];
   const double d2 = Yx * Yx + Yy * Yy;
@@ -160,31 +232,41 @@ __device__ inline void consider(const CellArgs &a, int64_t k, int32_t j, double 
   if (d2 - 2.0 * rho * d + wk - wj > 1e-12 * (d2 + fabs(wk) + fabs(wj))) return;
   for (int v = 0; v < n; v++)
     if (L.sl[v][L.t] == j) return;  // already an edge of the cell
-  n = clip_lds(L.sx, L.sy, L.sl, L.t, n, Yx, Yy, d2 + wk - wj, j, fail);
+  HalfPlane<R> H;
+  H.Yx64 = Yx;
+  H.Yy64 = Yy;
+  H.kk64 = d2 + wk - wj;
+  H.Yx = (R)Yx;
+  H.Yy = (R)Yy;
+  H.kk = (R)H.kk64;
+  H.label = j;
+  n = clip_lds(L, n, H, fail, nredo);
   rho2 = L.rho2(n);
 }
 
 // Writes a finished cell: polygon, bounding box, candidate triangle count.
+template <class R>
 __device__ inline void write_cell(const CellArgs &a, int64_t k, double yx, double yy,
-                                  const CellLDS &L, int n) {
+                                  const CellLDS<R> &L, int n) {
   const int64_t N = a.n;
   a.pn[k] = n;
   if (n == 0) {
     a.tcount[k] = 0;
     return;
   }
+  R *px = CellBuf<R>::px(a), *py = CellBuf<R>::py(a), *eacc = CellBuf<R>::eacc(a);
   double bx0 = kInf, by0 = kInf, bx1 = -kInf, by1 = -kInf;
   for (int v = 0; v < n; v++) {
-    const double x = L.sx[v][L.t], y = L.sy[v][L.t];
-    a.px[v * N + k] = x;
-    a.py[v * N + k] = y;
+    const R x = L.sx[v][L.t], y = L.sy[v][L.t];
+    px[v * N + k] = x;
+    py[v * N + k] = y;
     a.plab[v * N + k] = L.sl[v][L.t];
-    a.eacc[v * N + k] = 0.0;
+    eacc[v * N + k] = R(0);
     a.ehit[v * N + k] = 0;
-    bx0 = fmin(bx0, x);
-    bx1 = fmax(bx1, x);
-    by0 = fmin(by0, y);
-    by1 = fmax(by1, y);
+    bx0 = fmin(bx0, (double)x);
+    bx1 = fmax(bx1, (double)x);
+    by0 = fmin(by0, (double)y);
+    by1 = fmax(by1, (double)y);
   }
   bx0 += yx; bx1 += yx; by0 += yy; by1 += yy;
   a.bbox[4 * k] = bx0;
@@ -204,17 +286,40 @@ __device__ inline void zero_cell_outputs(const CellArgs &a, int64_t k) {
   a.mom[2 * k + 1] = 0.0;
 }
 
+// Lower bound of pi_j - pi_k over the polygon (radius rho about y_k) for
+// every Dirac j at distance >= D from y_k with weight <= W; the polygon is
+// safe from such j when it is positive.
+__device__ inline bool far_enough(double D, double rho, double W, double wk) {
+  return D > rho && D * D - 2.0 * rho * D - (W - wk) > 1e-12 * (D * D + fabs(W) + fabs(wk));
+}
+
+// Counts the lanes of the wave with flag set: one atomic per wave, issued by
+// the lowest such lane.
+template <class T>
+__device__ inline void wave_count(T *dst, bool flag) {
+  const unsigned long long bal = __ballot(flag);
+  if (bal && (threadIdx.x & 63) == (unsigned)(__ffsll((long long)bal) - 1))
+    atomicAdd(dst, (T)__popcll(bal));
+}
+
 // Site-centric construction: the domain box is clipped by the Diracs of the
-// bins in Chebyshev rings around the Dirac's own bin.  After each ring the
-// cell is certified if every unvisited Dirac j (distance >= D, weight <= W)
-// satisfies D^2 - 2 rho D - (W - w_k) > 0, i.e. cannot cut the polygon.
-// Uncertified cells are marked pn = -2 for the bin-candidate construction.
-__global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const double *wmax,
-                                                           int rmax) {
-  __shared__ double sx[kMaxFast][kCellBlock];
-  __shared__ double sy[kMaxFast][kCellBlock];
+// bins in Chebyshev rings 0..rmax around the Dirac's own bin.  After each
+// ring the cell is final if every unvisited Dirac (distance >= D from y_k,
+// weight <= the global maximum) is far_enough.  Otherwise the remaining
+// Diracs are reached through the max-weight pyramid: a depth-first walk in
+// Morton order (stackless: children of node m are 4m..4m+3 one level down)
+// that skips nodes inside the visited square, prunes nodes that are
+// far_enough for their own maximum weight and clips by the Diracs of the
+// bins it reaches.  Pruning uses the polygon radius at the time of the test,
+// which only shrinks, so every pruned Dirac stays unable to cut the final
+// cell.
+template <class R>
+__global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const double *pyr,
+                                                           PyrOff off, int rmax) {
+  __shared__ R sx[kMaxFast][kCellBlock];
+  __shared__ R sy[kMaxFast][kCellBlock];
   __shared__ int sl[kMaxFast][kCellBlock];
-  CellLDS L{sx, sy, sl, (int)threadIdx.x};
+  CellLDS<R> L{sx, sy, sl, (int)threadIdx.x};
   const int64_t k = blockIdx.x * (int64_t)kCellBlock + threadIdx.x;
   if (k >= a.n) return;
   const Grid &g = a.g;
@@ -226,20 +331,21 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
     return;
   }
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1], wk = a.w[k];
-  L.rect(a.dom[0] - yx, a.dom[1] - yy, a.dom[2] - yx, a.dom[3] - yy);
-  int n = 4;
+  L.rect((R)(a.dom[0] - yx), (R)(a.dom[1] - yy), (R)(a.dom[2] - yx), (R)(a.dom[3] - yy));
+  int n = 4, nredo = 0;
   bool fail = false, certified = false;
   double rho2 = L.rho2(4);
-  const double W = *wmax;
+  const double W = pyr[off.o[g.L]];
   const int sbx = bin_coord(yx, g.x0, g.h, g.G), sby = bin_coord(yy, g.y0, g.h, g.G);
-  for (int r = 0; r <= rmax && n > 0 && !fail; r++) {
+  int r = 0;
+  for (; r <= rmax && n > 0 && !fail; r++) {
     for (int by = max(sby - r, 0); by <= min(sby + r, g.G - 1) && n > 0 && !fail; by++) {
       const bool edge_row = (by == sby - r) || (by == sby + r);
       for (int bx = max(sbx - r, 0); bx <= min(sbx + r, g.G - 1) && n > 0 && !fail; bx++) {
         if (!edge_row && bx != sbx - r && bx != sbx + r) continue;
         const int64_t b = morton2((uint32_t)bx, (uint32_t)by);
         for (int32_t q = a.bin_start[b]; q < a.bin_start[b + 1] && n > 0 && !fail; q++)
-          consider(a, k, q, yx, yy, wk, L, n, rho2, fail);
+          consider(a, k, q, yx, yy, wk, L, n, rho2, fail, nredo);
       }
     }
     if (n == 0 || fail) break;
@@ -248,40 +354,89 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
     if (sbx + r < g.G - 1) D = fmin(D, g.x0 + (sbx + r + 1) * g.h - yx);
     if (sby - r > 0) D = fmin(D, yy - (g.y0 + (sby - r) * g.h));
     if (sby + r < g.G - 1) D = fmin(D, g.y0 + (sby + r + 1) * g.h - yy);
-    const double rho = sqrt(rho2);
-    if (D == kInf ||
-        (D > rho && D * D - 2.0 * rho * D - (W - wk) > 1e-12 * (D * D + fabs(W) + fabs(wk)))) {
+    if (D == kInf || far_enough(D, sqrt(rho2), W, wk)) {
       certified = true;
       break;
     }
   }
+  const bool walk = !certified && n > 0 && !fail;
+  bool over_budget = false;
+  if (walk) {
+    // bins [sbx - rr, sbx + rr] x [sby - rr, sby + rr] have been visited
+    const int rr = min(r, rmax);
+    const int vx0 = sbx - rr, vx1 = sbx + rr, vy0 = sby - rr, vy1 = sby + rr;
+    int l = g.L;
+    uint32_t m = 0;
+    int visits = 0;
+    while (n > 0 && !fail) {
+      if (++visits > kWalkBudget) {
+        // weights with a large spread leave many nodes unprunable: the cell
+        // is rebuilt from the per-bin candidate owners instead
+        over_budget = true;
+        break;
+      }
+      const int nx0 = (int)(compact1by1(m) << l), ny0 = (int)(compact1by1(m >> 1) << l);
+      const int nx1 = nx0 + (1 << l) - 1, ny1 = ny0 + (1 << l) - 1;
+      bool descend = false;
+      if (!(nx0 >= vx0 && nx1 <= vx1 && ny0 >= vy0 && ny1 <= vy1)) {
+        const double Wn = pyr[off.o[l] + m];
+        if (Wn != -kInf) {
+          const double bx0 = g.x0 + nx0 * g.h, bx1 = g.x0 + (nx1 + 1) * g.h;
+          const double by0 = g.y0 + ny0 * g.h, by1 = g.y0 + (ny1 + 1) * g.h;
+          const double dx = fmax(fmax(bx0 - yx, yx - bx1), 0.0);
+          const double dy = fmax(fmax(by0 - yy, yy - by1), 0.0);
+          if (!far_enough(sqrt(dx * dx + dy * dy), sqrt(rho2), Wn, wk)) {
+            if (l == 0) {
+              for (int32_t q = a.bin_start[m]; q < a.bin_start[m + 1] && n > 0 && !fail; q++)
+                consider(a, k, q, yx, yy, wk, L, n, rho2, fail, nredo);
+            } else {
+              descend = true;
+            }
+          }
+        }
+      }
+      if (descend) {
+        l--;
+        m <<= 2;
+        continue;
+      }
+      while (l < g.L && (m & 3u) == 3u) {
+        m >>= 2;
+        l++;
+      }
+      if (l == g.L) break;
+      m++;
+    }
+  }
+  wave_count(&a.flags[2], walk);  // cells that needed the pyramid walk
+  wave_count(&a.counters[6], over_budget && !fail);  // cells left to the bin path
+  if (Tol<R>::redo > R(0) && nredo > 0) atomicAdd(&a.counters[5], (unsigned long long)nredo);
   if (fail) {
     a.pn[k] = -1;
     a.tcount[k] = 0;
     atomicOr(&a.flags[3], 1);
     return;
   }
-  if (!certified && n > 0) {
+  if (over_budget) {
     a.pn[k] = -2;
     a.tcount[k] = 0;
-    atomicOr(&a.flags[2], 1);
     return;
   }
   write_cell(a, k, yx, yy, L, n);
 }
 
-// Bin-candidate construction for the cells listed in `list` (those the
-// site-centric search could not certify): the bin rectangle of the Dirac is
-// clipped by the candidate owners of its bins, nearest rings first.  The
-// rectangle covers every bin where the Dirac may own a point and the owners
-// of every point of these bins are among the candidates, so no certificate
-// is needed.
+// Bin-candidate construction for the cells listed in `list` (those whose
+// pyramid walk ran over budget): the bin rectangle of the Dirac is clipped by
+// the candidate owners of its bins, nearest rings first.  The rectangle
+// covers every bin where the Dirac may own a point and the owners of every
+// point of these bins are among the candidates, so no certificate is needed.
+template <class R>
 __global__ void __launch_bounds__(kCellBlock) k_cell_bins(CellArgs a, const int32_t *list,
                                                           const int32_t *nlist) {
-  __shared__ double sx[kMaxFast][kCellBlock];
-  __shared__ double sy[kMaxFast][kCellBlock];
+  __shared__ R sx[kMaxFast][kCellBlock];
+  __shared__ R sy[kMaxFast][kCellBlock];
   __shared__ int sl[kMaxFast][kCellBlock];
-  CellLDS L{sx, sy, sl, (int)threadIdx.x};
+  CellLDS<R> L{sx, sy, sl, (int)threadIdx.x};
   const int64_t idx = blockIdx.x * (int64_t)kCellBlock + threadIdx.x;
   if (idx >= *nlist) return;
   const int64_t k = list[idx];
@@ -294,9 +449,9 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_bins(CellArgs a, const int3
     return;
   }
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1], wk = a.w[k];
-  L.rect(g.x0 + rx0 * g.h - yx, g.y0 + ry0 * g.h - yy, g.x0 + (rx1 + 1) * g.h - yx,
-         g.y0 + (ry1 + 1) * g.h - yy);
-  int n = 4;
+  L.rect((R)(g.x0 + rx0 * g.h - yx), (R)(g.y0 + ry0 * g.h - yy),
+         (R)(g.x0 + (rx1 + 1) * g.h - yx), (R)(g.y0 + (ry1 + 1) * g.h - yy));
+  int n = 4, nredo = 0;
   bool fail = false;
   double rho2 = L.rho2(4);
   const int sbx = min(max(bin_coord(yx, g.x0, g.h, g.G), rx0), rx1);
@@ -311,10 +466,11 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_bins(CellArgs a, const int3
         const int c = a.cand_cnt[b];
         const int32_t *cl = a.cand + b * (int64_t)a.cap;
         for (int i = 0; i < c && n > 0 && !fail; i++)
-          consider(a, k, cl[i], yx, yy, wk, L, n, rho2, fail);
+          consider(a, k, cl[i], yx, yy, wk, L, n, rho2, fail, nredo);
       }
     }
   }
+  if (Tol<R>::redo > R(0) && nredo > 0) atomicAdd(&a.counters[5], (unsigned long long)nredo);
   if (fail) {
     a.pn[k] = -1;
     a.tcount[k] = 0;
@@ -342,16 +498,18 @@ __global__ void k_pairs(CellArgs a, const int32_t *poff, int32_t *pair_cell, int
 // Streaming Sutherland-Hodgman stage: clips by the closed half-plane left of
 // a -> b.  Each vertex carries the label of its outgoing edge.
 // ---------------------------------------------------------------------------
+template <class R>
 struct SV {
-  double x, y;
+  R x, y;
   int lab;
 };
+template <class R>
 struct Stage {
-  double ax, ay, ex, ey, elen;
+  R ax, ay, ex, ey, elen;
   bool started;
-  SV first, prev;
-  double fs, ps;
-  __device__ void init(double x0, double y0, double x1, double y1) {
+  SV<R> first, prev;
+  R fs, ps;
+  __device__ void init(R x0, R y0, R x1, R y1) {
     ax = x0;
     ay = y0;
     ex = x1 - x0;
@@ -362,38 +520,39 @@ struct Stage {
   // Side of v w.r.t. the edge line; values within rounding of zero count as
   // on the line (inside: the triangle is closed), so a cell edge lying on a
   // mesh edge stays with the triangle on the cell's side.
-  __device__ double side(const SV &v) const {
-    const double dx = v.x - ax, dy = v.y - ay;
-    const double s = ex * dy - ey * dx;
-    return fabs(s) <= kSnap * elen * (fabs(v.x) + fabs(v.y) + fabs(dx) + fabs(dy)) ? 0.0 : s;
+  __device__ R side(const SV<R> &v) const {
+    const R dx = v.x - ax, dy = v.y - ay;
+    const R s = ex * dy - ey * dx;
+    return fabs(s) <= Tol<R>::snap * elen * (fabs(v.x) + fabs(v.y) + fabs(dx) + fabs(dy)) ? R(0)
+                                                                                          : s;
   }
-  __device__ static SV cross_pt(const SV &p, double sp, const SV &c, double sc, int lab) {
-    const double tt = sp / (sp - sc);
-    SV r;
+  __device__ static SV<R> cross_pt(const SV<R> &p, R sp, const SV<R> &c, R sc, int lab) {
+    const R tt = sp / (sp - sc);
+    SV<R> r;
     r.x = p.x + tt * (c.x - p.x);
     r.y = p.y + tt * (c.y - p.y);
     r.lab = lab;
     return r;
   }
   // edge p -> c: emits up to two vertices
-  __device__ int edge(const SV &p, double sp, const SV &c, double sc, SV &o0, SV &o1) const {
-    if (sp >= 0.0) {
-      if (sc >= 0.0) {
+  __device__ int edge(const SV<R> &p, R sp, const SV<R> &c, R sc, SV<R> &o0, SV<R> &o1) const {
+    if (sp >= R(0)) {
+      if (sc >= R(0)) {
         o0 = c;
         return 1;
       }
       o0 = cross_pt(p, sp, c, sc, -1);
       return 1;
     }
-    if (sc >= 0.0) {
+    if (sc >= R(0)) {
       o0 = cross_pt(p, sp, c, sc, p.lab);
       o1 = c;
       return 2;
     }
     return 0;
   }
-  __device__ int push(const SV &c, SV &o0, SV &o1) {
-    const double s = side(c);
+  __device__ int push(const SV<R> &c, SV<R> &o0, SV<R> &o1) {
+    const R s = side(c);
     if (!started) {
       started = true;
       first = c;
@@ -407,21 +566,46 @@ struct Stage {
     ps = s;
     return m;
   }
-  __device__ int flush(SV &o0, SV &o1) const {
+  __device__ int flush(SV<R> &o0, SV<R> &o1) const {
     if (!started) return 0;
     return edge(prev, ps, first, fs, o0, o1);
   }
 };
 
-template <int MODE>
+// s += v; FP32 sums carry a Kahan compensation c (the true sum is s - c).
+template <class R>
+__device__ inline void acc_add(R &s, R &c, R v) {
+  if (sizeof(R) == sizeof(float)) {
+    const R y = v - c;
+    const R t = s + y;
+    c = (t - s) - y;
+    s = t;
+  } else {
+    s += v;
+  }
+}
+
+// MODE 0: mass, int rho |x - y|^2 and Hessian edge integrals; MODE 1: mass
+// and first moments; MODE 2: MODE 1 plus, per triangle vertex v, the
+// integral of (|x - y_k|^2 - w_k) phi_v(x) over the fragment (phi_v: P1 hat
+// function of v, linear on the triangle), the derivative of the dual value
+// w.r.t. the density value at v.
+template <int MODE, class R>
 struct Integrator {
   // fragment vertices arrive in order; fan from the first one
   int cnt;
-  SV first, prev;
-  double area, mass, intg, mx, my;
-  double r0, ga, gb;  // rho(Dirac + X) = r0 + ga X + gb Y
+  SV<R> first, prev;
+  R area, mass, intg, mx, my;
+  R cmass, cintg;     // Kahan compensations (FP32)
+  // rho(Dirac + X) = r0 + ga (X - tx) + gb (Y - ty): the linear density
+  // anchored at the triangle's first vertex (tx, ty), so that its values
+  // inside the triangle carry no cancellation from extrapolating to the Dirac
+  R r0, ga, gb, tx, ty;
+  double wk;          // weight of the Dirac (MODE 2)
+  double la[3], lb[3], lc[3];  // barycentric coordinates la X + lb Y + lc (MODE 2)
+  double dv[3];
   // extent of the fragment, for the degeneracy test
-  double xlo, xhi, ylo, yhi, cmax;
+  R xlo, xhi, ylo, yhi, cmax;
   // A fragment counts only if its area is positive: the reference clips the
   // closed triangle by the open cell (ties excluded), so a cell touching a
   // triangle only along an edge or at a point yields nothing.  Exactly
@@ -430,23 +614,23 @@ struct Integrator {
   // known (named registers rather than arrays, so nothing spills to scratch).
   int nb;
   int bl0, bl1, bl2, bl3;
-  double bv0, bv1, bv2, bv3;
+  R bv0, bv1, bv2, bv3;
   __device__ bool positive() const {
-    return area > 1e-13 * ((xhi - xlo) + (yhi - ylo)) * cmax;
+    return area > Tol<R>::area * ((xhi - xlo) + (yhi - ylo)) * cmax;
   }
-  double *eacc;
+  R *eacc;
   int32_t *ehit;
   int64_t N, k;
-  __device__ void commit(int lab, double r) {
+  __device__ void commit(int lab, R r) {
     atomicAdd(&eacc[lab * N + k], r);
     ehit[lab * N + k] = 1;
   }
-  __device__ void edge(const SV &a, const SV &b) {
+  __device__ void edge(const SV<R> &a, const SV<R> &b) {
     if (MODE != 0 || a.lab < 0) return;
-    const double ex = b.x - a.x, ey = b.y - a.y;
-    if (ex == 0.0 && ey == 0.0) return;  // zero-length piece from a snapped vertex
-    const double r =
-        sqrt(ex * ex + ey * ey) * (r0 + ga * 0.5 * (a.x + b.x) + gb * 0.5 * (a.y + b.y));
+    const R ex = b.x - a.x, ey = b.y - a.y;
+    if (ex == R(0) && ey == R(0)) return;  // zero-length piece from a snapped vertex
+    const R r = sqrt(ex * ex + ey * ey) *
+                (r0 + ga * (R(0.5) * (a.x + b.x) - tx) + gb * (R(0.5) * (a.y + b.y) - ty));
     if (nb == 0) {
       bl0 = a.lab;
       bv0 = r;
@@ -467,20 +651,21 @@ struct Integrator {
     }
     nb++;
   }
-  __device__ void fan(const SV &p0, const SV &p1, const SV &p2) {
-    const double x0 = p0.x, y0 = p0.y, x1 = p1.x, y1 = p1.y, x2 = p2.x, y2 = p2.y;
-    const double ar = 0.5 * ((x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0));
-    const double gx = (x0 + x1 + x2) / 3.0, gy = (y0 + y1 + y2) / 3.0;
+  __device__ void fan(const SV<R> &p0, const SV<R> &p1, const SV<R> &p2) {
+    const R x0 = p0.x, y0 = p0.y, x1 = p1.x, y1 = p1.y, x2 = p2.x, y2 = p2.y;
+    const R ar = R(0.5) * ((x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0));
+    const R gx = (x0 + x1 + x2) * R(1.0 / 3.0), gy = (y0 + y1 + y2) * R(1.0 / 3.0);
     area += ar;
-    mass += ar * (r0 + ga * gx + gb * gy);
-    AC6 q;
+    acc_add(mass, cmass, ar * (r0 + ga * (gx - tx) + gb * (gy - ty)));
+    AC6T<R> q;
     ac6_points(x0, y0, x1, y1, x2, y2, q);
-    double a1 = 0.0, a9 = 0.0, b1 = 0.0, b9 = 0.0;
+    R a1 = R(0), a9 = R(0), b1 = R(0), b9 = R(0);
+    double c1[3] = {0.0, 0.0, 0.0}, c9[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int p = 0; p < 6; p++) {
-      const double rp = r0 + ga * q.px[p] + gb * q.py[p];
+      const R rp = r0 + ga * (q.px[p] - tx) + gb * (q.py[p] - ty);
       if (MODE == 0) {
-        const double f = rp * (q.px[p] * q.px[p] + q.py[p] * q.py[p]);
+        const R f = rp * (q.px[p] * q.px[p] + q.py[p] * q.py[p]);
         if (p < 3) a1 += f; else a9 += f;
       } else {
         if (p < 3) {
@@ -491,15 +676,27 @@ struct Integrator {
           b9 += rp * q.py[p];
         }
       }
+      if (MODE == 2) {
+        const double c = q.px[p] * q.px[p] + q.py[p] * q.py[p] - wk;
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+          const double f = c * (la[v] * q.px[p] + lb[v] * q.py[p] + lc[v]);
+          if (p < 3) c1[v] += f; else c9[v] += f;
+        }
+      }
     }
     if (MODE == 0) {
-      intg += ar * (a1 * (1.0 / 30.0) + a9 * (9.0 / 30.0));
+      acc_add(intg, cintg, ar * (a1 * R(1.0 / 30.0) + a9 * R(9.0 / 30.0)));
     } else {
-      mx += ar * (a1 * (1.0 / 30.0) + a9 * (9.0 / 30.0));
-      my += ar * (b1 * (1.0 / 30.0) + b9 * (9.0 / 30.0));
+      mx += ar * (a1 * R(1.0 / 30.0) + a9 * R(9.0 / 30.0));
+      my += ar * (b1 * R(1.0 / 30.0) + b9 * R(9.0 / 30.0));
+    }
+    if (MODE == 2) {
+#pragma unroll
+      for (int v = 0; v < 3; v++) dv[v] += ar * (c1[v] * (1.0 / 30.0) + c9[v] * (9.0 / 30.0));
     }
   }
-  __device__ void push(const SV &v) {
+  __device__ void push(const SV<R> &v) {
     xlo = fmin(xlo, v.x);
     xhi = fmax(xhi, v.x);
     ylo = fmin(ylo, v.y);
@@ -525,11 +722,10 @@ struct Integrator {
   }
 };
 
-template <int MODE>
+template <int MODE, class R>
 __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
                                               const int32_t *__restrict__ pair_cell,
-                                              const int32_t *__restrict__ pair_tri,
-                                              unsigned long long *counters) {
+                                              const int32_t *__restrict__ pair_tri) {
   const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (p >= npairs) return;
   const int64_t k = pair_cell[p];
@@ -537,39 +733,57 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   const int64_t N = a.n;
   const int n = a.pn[k];
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1];
-  Stage s1, s2, s3;
-  s1.init(t.x[0] - yx, t.y[0] - yy, t.x[1] - yx, t.y[1] - yy);
-  s2.init(t.x[1] - yx, t.y[1] - yy, t.x[2] - yx, t.y[2] - yy);
-  s3.init(t.x[2] - yx, t.y[2] - yy, t.x[0] - yx, t.y[0] - yy);
-  Integrator<MODE> I;
+  // triangle in coordinates local to the Dirac (FP64, then rounded once)
+  const R X0 = (R)(t.x[0] - yx), Y0 = (R)(t.y[0] - yy), X1 = (R)(t.x[1] - yx);
+  const R Y1 = (R)(t.y[1] - yy), X2 = (R)(t.x[2] - yx), Y2 = (R)(t.y[2] - yy);
+  Stage<R> s1, s2, s3;
+  s1.init(X0, Y0, X1, Y1);
+  s2.init(X1, Y1, X2, Y2);
+  s3.init(X2, Y2, X0, Y0);
+  Integrator<MODE, R> I;
   I.cnt = 0;
-  I.area = I.mass = I.intg = I.mx = I.my = 0.0;
-  I.r0 = t.a * yx + t.b * yy + t.c;
-  I.ga = t.a;
-  I.gb = t.b;
+  I.area = I.mass = I.intg = I.mx = I.my = R(0);
+  I.cmass = I.cintg = R(0);
+  I.r0 = (R)(t.a * t.x[0] + t.b * t.y[0] + t.c);  // the value at vertex 0
+  I.ga = (R)t.a;
+  I.gb = (R)t.b;
+  I.tx = X0;
+  I.ty = Y0;
   I.nb = 0;
-  I.xlo = I.ylo = kInf;
-  I.xhi = I.yhi = -kInf;
-  I.cmax = 0.0;
-  I.eacc = a.eacc;
+  I.xlo = I.ylo = R(kInf);
+  I.xhi = I.yhi = R(-kInf);
+  I.cmax = R(0);
+  I.eacc = CellBuf<R>::eacc(a);
   I.ehit = a.ehit;
   I.N = N;
   I.k = k;
-  SV o0, o1, q0, q1, r0, r1;
-  auto to3 = [&](const SV &v) {
+  if (MODE == 2) {
+    // barycentric coordinates of the triangle in coordinates local to the Dirac
+    const double X0 = t.x[0] - yx, Y0 = t.y[0] - yy, X1 = t.x[1] - yx, Y1 = t.y[1] - yy;
+    const double X2 = t.x[2] - yx, Y2 = t.y[2] - yy;
+    const double inv = 1.0 / ((X1 - X0) * (Y2 - Y0) - (X2 - X0) * (Y1 - Y0));
+    I.la[0] = (Y1 - Y2) * inv; I.lb[0] = (X2 - X1) * inv; I.lc[0] = (X1 * Y2 - X2 * Y1) * inv;
+    I.la[1] = (Y2 - Y0) * inv; I.lb[1] = (X0 - X2) * inv; I.lc[1] = (X2 * Y0 - X0 * Y2) * inv;
+    I.la[2] = (Y0 - Y1) * inv; I.lb[2] = (X1 - X0) * inv; I.lc[2] = (X0 * Y1 - X1 * Y0) * inv;
+    I.wk = a.w[k];
+    I.dv[0] = I.dv[1] = I.dv[2] = 0.0;
+  }
+  const R *px = CellBuf<R>::px(a), *py = CellBuf<R>::py(a);
+  SV<R> o0, o1, q0, q1, r0, r1;
+  auto to3 = [&](const SV<R> &v) {
     const int m3 = s3.push(v, r0, r1);
     if (m3 > 0) I.push(r0);
     if (m3 > 1) I.push(r1);
   };
-  auto to2 = [&](const SV &v) {
+  auto to2 = [&](const SV<R> &v) {
     const int m2 = s2.push(v, q0, q1);
     if (m2 > 0) to3(q0);
     if (m2 > 1) to3(q1);
   };
   for (int v = 0; v < n; v++) {
-    SV c;
-    c.x = a.px[v * N + k];
-    c.y = a.py[v * N + k];
+    SV<R> c;
+    c.x = px[v * N + k];
+    c.y = py[v * N + k];
     c.lab = (a.plab[v * N + k] >= 0) ? v : -1;
     const int m1 = s1.push(c, o0, o1);
     if (m1 > 0) to2(o0);
@@ -587,90 +801,102 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
     if (m3 > 1) I.push(r1);
     I.finish();
   }
-  if (I.cnt >= 3 && I.positive()) {
+  const bool counted = I.cnt >= 3 && I.positive();
+  if (counted) {
     if (MODE == 0) {
-      atomicAdd(&a.mass[k], I.mass);
-      atomicAdd(&a.intg[k], I.intg);
+      atomicAdd(&a.mass[k], (double)I.mass - (double)I.cmass);
+      atomicAdd(&a.intg[k], (double)I.intg - (double)I.cintg);
     } else {
-      atomicAdd(&a.mass[k], I.mass);
-      atomicAdd(&a.mom[2 * k], I.mx);
-      atomicAdd(&a.mom[2 * k + 1], I.my);
+      atomicAdd(&a.mass[k], (double)I.mass);
+      atomicAdd(&a.mom[2 * k], (double)I.mx);
+      atomicAdd(&a.mom[2 * k + 1], (double)I.my);
     }
-    atomicAdd(&counters[0], 1ull);
+    if (MODE == 2) {
+      const int64_t ti = pair_tri[p];
+#pragma unroll
+      for (int v = 0; v < 3; v++) atomicAdd(&a.dval[a.triv[3 * ti + v]], I.dv[v]);
+    }
   }
+  wave_count(&a.counters[0], counted);
 }
 
 // Hessian row of every fast-path cell (include/MA/kantorovich.hpp:109-122):
-// H_kj = -int_{e_kj} rho ds / (2 |y_k - y_j|), H_kk = -sum_j H_kj.
+// H_kj = -int_{e_kj} rho ds / (2 |y_k - y_j|), H_kk = -sum_j H_kj, in FP64.
+template <class R>
 __global__ void k_rows(CellArgs a, double *fval_part, int32_t *row_cnt, int32_t *row_col,
-                       double *row_val, int cap_row, unsigned long long *counters) {
+                       double *row_val, int cap_row) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int n = (k < a.n) ? a.pn[k] : -1;  // n < 0: the general kernel writes this row
-  unsigned long long my_max = 0, my_empty = 0;
+  unsigned long long my_max = 0, my_empty = 0, my_nv = 0;
   if (n >= 0) {
-  const int64_t N = a.n;
-  const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1];
-  int32_t cols[kMaxFast + 1];
-  double vals[kMaxFast + 1];
-  int nr = 0;
-  double diag = 0.0;
-  for (int v = 0; v < n; v++) {
-    const int32_t j = a.plab[v * N + k];
-    if (j < 0 || !a.ehit[v * N + k]) continue;
-    const double dx = yx - a.pos[2 * j], dy = yy - a.pos[2 * j + 1];
-    const double val = a.eacc[v * N + k] / (2.0 * sqrt(dx * dx + dy * dy));
-    diag += val;
-    int p = 0;
-    while (p < nr && cols[p] < j) p++;
-    if (p < nr && cols[p] == j) {
-      vals[p] -= val;
-      continue;
+    const int64_t N = a.n;
+    const R *eacc = CellBuf<R>::eacc(a);
+    const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1];
+    int32_t cols[kMaxFast + 1];
+    double vals[kMaxFast + 1];
+    int nr = 0;
+    double diag = 0.0;
+    for (int v = 0; v < n; v++) {
+      const int32_t j = a.plab[v * N + k];
+      if (j < 0 || !a.ehit[v * N + k]) continue;
+      const double dx = yx - a.pos[2 * j], dy = yy - a.pos[2 * j + 1];
+      const double val = (double)eacc[v * N + k] / (2.0 * sqrt(dx * dx + dy * dy));
+      diag += val;
+      int p = 0;
+      while (p < nr && cols[p] < j) p++;
+      if (p < nr && cols[p] == j) {
+        vals[p] -= val;
+        continue;
+      }
+      for (int r = nr; r > p; r--) {
+        cols[r] = cols[r - 1];
+        vals[r] = vals[r - 1];
+      }
+      cols[p] = j;
+      vals[p] = -val;
+      nr++;
     }
-    for (int r = nr; r > p; r--) {
-      cols[r] = cols[r - 1];
-      vals[r] = vals[r - 1];
-    }
-    cols[p] = j;
-    vals[p] = -val;
-    nr++;
-  }
-  int cnt = 0;
-  if (nr > 0) {
-    int32_t *oc = row_col + k * (int64_t)cap_row;
-    double *ov = row_val + k * (int64_t)cap_row;
-    bool placed = false;
-    for (int r = 0; r < nr; r++) {
-      if (!placed && cols[r] > k) {
+    int cnt = 0;
+    if (nr > 0) {
+      int32_t *oc = row_col + k * (int64_t)cap_row;
+      double *ov = row_val + k * (int64_t)cap_row;
+      bool placed = false;
+      for (int r = 0; r < nr; r++) {
+        if (!placed && cols[r] > k) {
+          oc[cnt] = (int32_t)k;
+          ov[cnt++] = diag;
+          placed = true;
+        }
+        oc[cnt] = cols[r];
+        ov[cnt++] = vals[r];
+      }
+      if (!placed) {
         oc[cnt] = (int32_t)k;
         ov[cnt++] = diag;
-        placed = true;
       }
-      oc[cnt] = cols[r];
-      ov[cnt++] = vals[r];
     }
-    if (!placed) {
-      oc[cnt] = (int32_t)k;
-      ov[cnt++] = diag;
-    }
+    row_cnt[k] = cnt;
+    const double m = a.mass[k];
+    fval_part[k] = a.w[k] * m - a.intg[k];
+    my_max = (unsigned long long)n;
+    my_empty = (m <= 0.0) ? 1ull : 0ull;
+    my_nv = (unsigned long long)n;
   }
-  row_cnt[k] = cnt;
-  const double m = a.mass[k];
-  fval_part[k] = a.w[k] * m - a.intg[k];
-  my_max = (unsigned long long)n;
-  my_empty = (m <= 0.0) ? 1ull : 0ull;
-  }
-  __shared__ unsigned long long s_max, s_empty;
+  __shared__ unsigned long long s_max, s_empty, s_nv;
   if (threadIdx.x == 0) {
     s_max = 0;
     s_empty = 0;
+    s_nv = 0;
   }
   __syncthreads();
   atomicMax(&s_max, my_max);
   atomicAdd(&s_empty, my_empty);
+  atomicAdd(&s_nv, my_nv);
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicMax(&counters[2], s_max);
-    atomicAdd(&counters[3], s_empty);
+    atomicMax(&a.counters[2], s_max);
+    atomicAdd(&a.counters[3], s_empty);
+    atomicAdd(&a.counters[4], s_nv);
   }
 }
 
@@ -697,12 +923,22 @@ namespace ma {
 
 ma_status run_bin_candidates(ma_engine *e, const int32_t *pn);
 
-ma_status run_cells(ma_engine *e, int mode, int32_t *npairs_out, int32_t *ngeneral_out) {
+// mode: 0 Kantorovich, 1 first moments, 2 adjoint; fp32: FP32 geometry for
+// mode 0 (mixed precision).
+ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
+                    int32_t *ngeneral_out) {
   EvalBuffers &B = e->eb;
   const int64_t n = e->n;
-  MA_HIP_TRY(B.px.ensure((size_t)kMaxFast * n));
-  MA_HIP_TRY(B.py.ensure((size_t)kMaxFast * n));
-  MA_HIP_TRY(B.eacc.ensure((size_t)kMaxFast * n));
+  if (mode != 0) fp32 = false;
+  if (fp32) {
+    MA_HIP_TRY(B.pxf.ensure((size_t)kMaxFast * n));
+    MA_HIP_TRY(B.pyf.ensure((size_t)kMaxFast * n));
+    MA_HIP_TRY(B.eaccf.ensure((size_t)kMaxFast * n));
+  } else {
+    MA_HIP_TRY(B.px.ensure((size_t)kMaxFast * n));
+    MA_HIP_TRY(B.py.ensure((size_t)kMaxFast * n));
+    MA_HIP_TRY(B.eacc.ensure((size_t)kMaxFast * n));
+  }
   MA_HIP_TRY(B.plab.ensure((size_t)kMaxFast * n));
   MA_HIP_TRY(B.ehit.ensure((size_t)kMaxFast * n));
   MA_HIP_TRY(B.pn.ensure((size_t)n));
@@ -725,6 +961,8 @@ ma_status run_cells(ma_engine *e, int mode, int32_t *npairs_out, int32_t *ngener
   a.tri = e->tri.p;
   a.px = B.px.p;
   a.py = B.py.p;
+  a.pxf = B.pxf.p;
+  a.pyf = B.pyf.p;
   a.plab = B.plab.p;
   a.pn = B.pn.p;
   a.bbox = B.bbox.p;
@@ -733,39 +971,55 @@ ma_status run_cells(ma_engine *e, int mode, int32_t *npairs_out, int32_t *ngener
   a.intg = B.intg_s.p;
   a.mom = B.mom_s.p;
   a.eacc = B.eacc.p;
+  a.eaccf = B.eaccf.p;
   a.ehit = B.ehit.p;
   a.flags = B.flags.p;
+  a.counters = B.counters.p;
   a.force_general = e->force_general;
   a.bin_start = e->bin_start.p;
+  a.triv = e->triv.p;
+  a.dval = B.dval.p;
   for (int i = 0; i < 4; i++) a.dom[i] = e->dom[i];
   const int32_t zero = 0;
-  hipLaunchKernelGGL(k_cell_local, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0, e->stream,
-                     a, B.pyr.p + e->pyr_off[e->grid.L], kLocalRings);
+  PyrOff po{};
+  for (int l = 0; l <= e->grid.L; l++) po.o[l] = e->pyr_off[l];
+  if (fp32)
+    hipLaunchKernelGGL(k_cell_local<float>, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0,
+                       e->stream, a, B.pyr.p, po, kLocalRings);
+  else
+    hipLaunchKernelGGL(k_cell_local<double>, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0,
+                       e->stream, a, B.pyr.p, po, kLocalRings);
   {
     int32_t fl0[4];
+    unsigned long long ct0[8];
     MA_HIP_TRY(hipMemcpyAsync(fl0, B.flags.p, sizeof fl0, hipMemcpyDeviceToHost, e->stream));
+    MA_HIP_TRY(hipMemcpyAsync(ct0, B.counters.p, sizeof ct0, hipMemcpyDeviceToHost, e->stream));
     MA_HIP_TRY(hipStreamSynchronize(e->stream));
-    e->info.nbinpath = 0;
-    if (fl0[2] || fl0[3]) {
-      // cells the site-centric search could not certify: bin-candidate path
+    e->info.nbinpath = fl0[2];
+    e->ran_candidates = false;
+    if (fl0[3] || ct0[6]) {
+      // cells over the walk budget or overflowing the fast path: per-bin
+      // candidate owners (and bin rectangles of those cells)
       MA_HIP_TRY(hipEventRecord(e->ev[8], e->stream));
       ma_status s = run_bin_candidates(e, B.pn.p);
       if (s != MA_OK) return s;
-      a.cand = B.cand.p;
+      a.cand = B.cand.p;  // the lists may have been regrown
       a.cap = e->cap_cand;
-      MA_HIP_TRY(hipMemcpyAsync(B.glist.p + n, &zero, sizeof(int32_t), hipMemcpyHostToDevice,
-                                e->stream));
-      hipLaunchKernelGGL(k_collect, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, n, B.pn.p, -2,
-                         B.glist.p, B.glist.p + n);
-      int32_t nb = 0;
-      MA_HIP_TRY(hipMemcpyAsync(&nb, B.glist.p + n, sizeof(int32_t), hipMemcpyDeviceToHost,
-                                e->stream));
-      MA_HIP_TRY(hipStreamSynchronize(e->stream));
-      e->info.nbinpath = nb;
-      if (nb > 0)
-        hipLaunchKernelGGL(k_cell_bins, dim3(nblocks(nb, kCellBlock)), dim3(kCellBlock), 0,
-                           e->stream, a, B.glist.p, B.glist.p + n);
+      if (ct0[6]) {
+        MA_HIP_TRY(hipMemcpyAsync(B.glist.p + n, &zero, sizeof(int32_t), hipMemcpyHostToDevice,
+                                  e->stream));
+        hipLaunchKernelGGL(k_collect, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, n, B.pn.p,
+                           -2, B.glist.p, B.glist.p + n);
+        const unsigned nb = (unsigned)((ct0[6] + kCellBlock - 1) / kCellBlock);
+        if (fp32)
+          hipLaunchKernelGGL(k_cell_bins<float>, dim3(nb), dim3(kCellBlock), 0, e->stream, a,
+                             B.glist.p, B.glist.p + n);
+        else
+          hipLaunchKernelGGL(k_cell_bins<double>, dim3(nb), dim3(kCellBlock), 0, e->stream, a,
+                             B.glist.p, B.glist.p + n);
+      }
       MA_HIP_TRY(hipEventRecord(e->ev[9], e->stream));
+      e->ran_candidates = true;
     }
   }
   MA_HIP_TRY(hipEventRecord(e->ev[3], e->stream));
@@ -783,18 +1037,25 @@ ma_status run_cells(ma_engine *e, int mode, int32_t *npairs_out, int32_t *ngener
                      B.pair_cell.p, B.pair_tri.p);
   MA_HIP_TRY(hipEventRecord(e->ev[4], e->stream));
   if (np > 0) {
-    if (mode == 0)
-      hipLaunchKernelGGL(k_frag<0>, dim3(nblocks(np, 256)), dim3(256), 0, e->stream, a,
-                         (int64_t)np, B.pair_cell.p, B.pair_tri.p, B.counters.p);
+    const dim3 grid(nblocks(np, 256)), block(256);
+    const int32_t *pc = B.pair_cell.p, *pt = B.pair_tri.p;
+    if (mode == 0 && fp32)
+      hipLaunchKernelGGL((k_frag<0, float>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
+    else if (mode == 0)
+      hipLaunchKernelGGL((k_frag<0, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
+    else if (mode == 1)
+      hipLaunchKernelGGL((k_frag<1, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
     else
-      hipLaunchKernelGGL(k_frag<1>, dim3(nblocks(np, 256)), dim3(256), 0, e->stream, a,
-                         (int64_t)np, B.pair_cell.p, B.pair_tri.p, B.counters.p);
+      hipLaunchKernelGGL((k_frag<2, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
   }
   MA_HIP_TRY(hipEventRecord(e->ev[5], e->stream));
-  if (mode == 0)
-    hipLaunchKernelGGL(k_rows, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a, B.fval_part.p,
-                       B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row, B.counters.p);
-  else
+  if (mode == 0 && fp32)
+    hipLaunchKernelGGL(k_rows<float>, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a,
+                       B.fval_part.p, B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row);
+  else if (mode == 0)
+    hipLaunchKernelGGL(k_rows<double>, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a,
+                       B.fval_part.p, B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row);
+  else if (mode == 1)  // mode 2 keeps the moments about the Dirac
     hipLaunchKernelGGL(k_mom_finalize, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a);
   int32_t ng = 0;
   if (fl[3]) {

@@ -20,6 +20,14 @@ template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  ~DBuf() { release(); }
   hipError_t ensure(size_t count) {
     if (count <= n) return hipSuccess;
     if (p) {
@@ -68,12 +76,12 @@ struct EvalBuffers {
   DBuf<int32_t> rect;        // per Dirac: bin rectangle (x0, y0, x1, y1)
   DBuf<double> mass_s, intg_s, fval_part;
   DBuf<double> mom_s;        // first moments (2 per Dirac)
+  DBuf<double> dval;         // adjoint w.r.t. vertex density values (nv)
   DBuf<int32_t> row_cnt;     // per Dirac, entries incl. diagonal
   DBuf<int32_t> row_col;     // n x cap_row
   DBuf<double> row_val;      // n x cap_row
   DBuf<int32_t> csr_ptr, csr_col;   // Morton-order CSR
   DBuf<double> csr_val;
-  DBuf<float> csr_val32;
   DBuf<double> diag;
   DBuf<int32_t> flags;       // [0] overflow poly, [1] overflow cand, [2] overflow row
   DBuf<unsigned long long> counters;  // [0] fragments, [1] candidates, [2] max poly, [3] empty
@@ -81,12 +89,14 @@ struct EvalBuffers {
   DBuf<double> red_tmp;
   // register/LDS fast path
   DBuf<double> px, py, eacc, bbox;
+  DBuf<float> pxf, pyf, eaccf;   // FP32 polygons and edge integrals (mixed mode)
   DBuf<int32_t> plab, ehit, pn, tcount, poff, pair_cell, pair_tri, glist;
 };
 
 struct SolverBuffers {
   DBuf<double> w, m, nu, g, d, x, r, z, p, Ap, wtrial, red;
-  DBuf<float> r32, z32, p32, Ap32, x32;
+  DBuf<int32_t> hptr, hcol;   // caller-supplied CSR (ma_solve_laplacian_csr)
+  DBuf<double> hval, hg, hd;
 };
 
 }  // namespace ma
@@ -119,6 +129,8 @@ struct ma_engine {
   ma::Amg *amg = nullptr;
   int cap_cand = 48;
   int force_general = 0;  // MA_OPT_FORCE_GENERAL
+  int use_graphs = 1;     // MA_OPT_GRAPHS: replay CG batches from a HIP graph
+  bool ran_candidates = false;  // the last evaluation built per-bin candidate lists
   int cap_row = ma::kMaxPoly + 1;
 
   ma::EvalBuffers eb;
@@ -137,18 +149,26 @@ namespace ma {
 hipError_t build_dirac_grid(ma_engine *e, const double *d_X, int layout);
 hipError_t build_triangle_bins(ma_engine *e);
 // Evaluates masses / value / Hessian (CSR in Morton order) at Morton-ordered
-// weights e->eb.w_s.  mode 0: Kantorovich, mode 1: first moments.
-ma_status evaluate(ma_engine *e, int mode, double *fval_out);
+// weights e->eb.w_s.  mode 0: Kantorovich, mode 1: first moments, mode 2:
+// first moments and the density-value adjoint (into eb.dval, which the caller
+// zeroes).  fp32: FP32 cell geometry and quadrature (mixed precision, mode 0).
+ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32 = false);
 // Copy helpers between caller order and Morton order.
 hipError_t to_morton(ma_engine *e, const double *src_caller, double *dst_morton, bool src_on_device);
 hipError_t from_morton(ma_engine *e, const double *src_morton, double *dst_caller, bool dst_on_device);
 hipError_t exclusive_scan(ma_engine *e, const int32_t *in, int32_t *out, int64_t n);
 hipError_t csr_to_caller(ma_engine *e, int32_t *rowptr, int32_t *col, double *val);
 ma_status adjoint(ma_engine *e, double *dY_caller, double *dval);
+// A square CSR matrix on the device.
+struct CsrView {
+  int64_t n = 0, nnz = 0;
+  const int32_t *ptr = nullptr, *col = nullptr;
+  const double *val = nullptr;
+};
 // Multigrid preconditioner (amg.hip)
 Amg *amg_create();
 void amg_destroy(Amg *a);
-ma_status amg_setup(ma_engine *e, Amg &A, int32_t gi, int max_levels);
+ma_status amg_setup(ma_engine *e, Amg &A, const CsrView &M, int32_t gi, int max_levels);
 hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const double *done);
 const double *amg_level0_values(const Amg &a);
 const double *amg_level0_dinv(const Amg &a);

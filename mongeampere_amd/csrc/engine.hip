@@ -277,22 +277,17 @@ static hipError_t newton_stats(ma_engine *e, const double *m, const double *nu, 
   return hipSuccess;
 }
 
-// Solve H0 d = b on the gauge-fixed Hessian of the last evaluation (Morton
-// order) by the conjugate gradient, preconditioned by the aggregation
-// multigrid cycle (precond 0) or by Jacobi (precond 1).
-static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, int maxiter,
-                      int precond, int64_t *iters, double *rel_res) {
+// Solve M0 d = b by the conjugate gradient, M0 = M with row and column gi
+// replaced by the identity (the gauge), preconditioned by the aggregation
+// multigrid cycle (precond 0) or by Jacobi (precond 1).  Iterations run in
+// batches; the scalars stay on the device and the host checks the converged
+// flag once per batch.  With graphs enabled a batch is recorded once into a
+// HIP graph (kernel nodes only) and replayed.
+static hipError_t pcg(ma_engine *e, const CsrView &M, int32_t gi, const double *b, double *x,
+                      double rel_tol, int maxiter, int precond, int64_t *iters,
+                      double *rel_res) {
   SolverBuffers &S = e->sb;
-  EvalBuffers &B = e->eb;
-  const int64_t n = e->n;
-  int32_t gi = 0;
-  {
-    // gauge: the caller's last Dirac (its weight is never moved,
-    // include/MA/optimal_transport.hpp:82-86)
-    MA_HIP_RET(hipMemcpyAsync(&gi, e->iperm.p + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
-                              e->stream));
-    MA_HIP_RET(hipStreamSynchronize(e->stream));
-  }
+  const int64_t n = M.n;
   MA_HIP_RET(S.r.ensure((size_t)n));
   MA_HIP_RET(S.z.ensure((size_t)n));
   MA_HIP_RET(S.p.ensure((size_t)n));
@@ -303,10 +298,10 @@ static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, 
   if (!e->amg) e->amg = amg_create();
   {
     const int levels = precond == 0 ? 0 : 1;  // Jacobi needs level 0 only
-    ma_status st = amg_setup(e, *e->amg, gi, levels);
+    ma_status st = amg_setup(e, *e->amg, M, gi, levels);
     if (st != MA_OK) return hipErrorUnknown;
   }
-  const int32_t *ptr = B.csr_ptr.p, *col = B.csr_col.p;
+  const int32_t *ptr = M.ptr, *col = M.col;
   const double *val = amg_level0_values(*e->amg), *dinv = amg_level0_dinv(*e->amg);
   auto precondition = [&]() -> hipError_t {
     if (precond == 0) return amg_apply(e, *e->amg, S.r.p, S.z.p, scal + 5);
@@ -321,7 +316,6 @@ static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, 
                      part_a, scal);
   hipLaunchKernelGGL(k_cg_scal_init, dim3(1), dim3(kVB), 0, e->stream, part_a, part_b, scal);
   const double tol2 = rel_tol * rel_tol;
-  int it = 0;
   double sc[7];
   MA_HIP_RET(hipMemcpyAsync(sc, scal, sizeof sc, hipMemcpyDeviceToHost, e->stream));
   MA_HIP_RET(hipStreamSynchronize(e->stream));
@@ -330,10 +324,11 @@ static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, 
     *rel_res = 0.0;
     return hipSuccess;
   }
-  const int batch = precond == 0 ? 8 : 32;
-  while (it < maxiter) {
-    for (int q = 0; q < batch && it < maxiter; q++, it++) {
-      const int parity = it & 1;
+  const int batch = precond == 0 ? 8 : 32;  // even: the parity pattern repeats
+  // iterations it0 .. it0 + count - 1
+  auto launch = [&](int it0, int count) -> hipError_t {
+    for (int q = 0; q < count; q++) {
+      const int parity = (it0 + q) & 1;
       hipLaunchKernelGGL(k_spmv_dot, dim3(kVGrid), dim3(kVB), 0, e->stream, n, ptr, col, val,
                          S.p.p, S.Ap.p, part_c, scal);
       hipLaunchKernelGGL(k_cg_update, dim3(kVGrid), dim3(kVB), 0, e->stream, n, parity, part_c,
@@ -344,15 +339,54 @@ static hipError_t pcg(ma_engine *e, const double *b, double *x, double rel_tol, 
       hipLaunchKernelGGL(k_cg_dir, dim3(kVGrid), dim3(kVB), 0, e->stream, n, parity, part_a,
                          part_b, scal, tol2, S.z.p, S.p.p);
     }
-    MA_HIP_RET(hipGetLastError());
-    MA_HIP_RET(hipMemcpyAsync(sc, scal, sizeof sc, hipMemcpyDeviceToHost, e->stream));
-    MA_HIP_RET(hipStreamSynchronize(e->stream));
-    if (sc[5] != 0.0) break;
+    return hipGetLastError();
+  };
+  hipGraphExec_t exec = nullptr;
+  if (e->use_graphs && maxiter >= batch) {
+    hipGraph_t graph = nullptr;
+    MA_HIP_RET(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const hipError_t le = launch(0, batch);
+    const hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+    if (le != hipSuccess) return le;
+    MA_HIP_RET(ce);
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MA_HIP_RET(ie);
   }
+  int it = 0;
+  hipError_t err = hipSuccess;
+  while (it < maxiter) {
+    const int count = std::min(batch, maxiter - it);
+    if (exec && count == batch)
+      err = hipGraphLaunch(exec, e->stream);
+    else
+      err = launch(it, count);
+    if (err != hipSuccess) break;
+    it += count;
+    err = hipMemcpyAsync(sc, scal, sizeof sc, hipMemcpyDeviceToHost, e->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(e->stream);
+    if (err != hipSuccess || sc[5] != 0.0) break;
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  MA_HIP_RET(err);
   hipLaunchKernelGGL(k_set_zero_at, dim3(1), dim3(1), 0, e->stream, x, gi);
   *iters = (int64_t)sc[6];
   *rel_res = std::sqrt(sc[2] / sc[4]);
   return hipGetLastError();
+}
+
+// The Hessian of the last evaluation (Morton order) and its gauge slot: the
+// caller's last Dirac, whose weight is never moved
+// (include/MA/optimal_transport.hpp:82-86).
+static hipError_t eval_csr(ma_engine *e, CsrView &M, int32_t &gi) {
+  M.n = e->n;
+  M.nnz = e->nnz;
+  M.ptr = e->eb.csr_ptr.p;
+  M.col = e->eb.csr_col.p;
+  M.val = e->eb.csr_val.p;
+  MA_HIP_RET(hipMemcpyAsync(&gi, e->iperm.p + (e->n - 1), sizeof(int32_t), hipMemcpyDeviceToHost,
+                            e->stream));
+  return hipStreamSynchronize(e->stream);
 }
 
 }  // namespace ma
@@ -447,28 +481,7 @@ void ma_destroy(ma_engine *e) {
   for (auto &ev : e->ev) hipEventDestroy(ev);
   hipStreamDestroy(e->stream);
   if (e->amg) amg_destroy(e->amg);
-  e->tri.release();
-  e->triv.release();
-  e->vxy.release();
-  e->vval.release();
-  e->pos.release();
-  e->perm.release();
-  e->iperm.release();
-  e->bin_start.release();
-  e->tbin_start.release();
-  e->tbin_items.release();
-  EvalBuffers &B = e->eb;
-  B.w_s.release(); B.pyr.release(); B.cand_cnt.release(); B.cand.release(); B.owner.release();
-  B.rect.release(); B.mass_s.release(); B.intg_s.release(); B.fval_part.release();
-  B.mom_s.release(); B.row_cnt.release(); B.row_col.release(); B.row_val.release();
-  B.csr_ptr.release(); B.csr_col.release(); B.csr_val.release(); B.csr_val32.release();
-  B.diag.release(); B.flags.release(); B.counters.release(); B.scan_tmp.release();
-  B.red_tmp.release();
-  SolverBuffers &S = e->sb;
-  for (DBuf<double> *b : {&S.w, &S.m, &S.nu, &S.g, &S.d, &S.x, &S.r, &S.z, &S.p, &S.Ap,
-                          &S.wtrial, &S.red})
-    b->release();
-  for (DBuf<float> *b : {&S.r32, &S.z32, &S.p32, &S.Ap32, &S.x32}) b->release();
+  // every device buffer is released by its owner's destructor
   delete e;
 }
 
@@ -556,8 +569,11 @@ ma_status ma_set_diracs(ma_engine *e, int64_t n, const double *X, int layout) {
 }
 
 static ma_status eval_morton(ma_engine *e, int precision, double *fval) {
-  (void)precision;
-  return evaluate(e, 0, fval);
+  if (precision != MA_FP64 && precision != MA_MIXED) {
+    e->err = "unknown precision";
+    return MA_BAD_INPUT;
+  }
+  return evaluate(e, 0, fval, precision == MA_MIXED);
 }
 
 ma_status ma_kantorovich(ma_engine *e, const double *w, int precision, double *fval, double *g,
@@ -646,9 +662,64 @@ ma_status ma_solve_laplacian(ma_engine *e, const double *g, double *d, double re
   MA_HIP_TRY(to_morton(e, g, S.g.p, false));
   int64_t it = 0;
   double rr = 0.0;
-  MA_HIP_TRY(pcg(e, S.g.p, S.d.p, rel_tol > 0 ? rel_tol : 1e-12,
+  CsrView M;
+  int32_t gi = 0;
+  MA_HIP_TRY(eval_csr(e, M, gi));
+  MA_HIP_TRY(pcg(e, M, gi, S.g.p, S.d.p, rel_tol > 0 ? rel_tol : 1e-12,
                  maxiter > 0 ? maxiter : 100000, 0, &it, &rr));
   MA_HIP_TRY(from_morton(e, S.d.p, d, false));
+  if (res) *res = rr;
+  if (iters) *iters = it;
+  return MA_OK;
+}
+
+ma_status ma_solve_laplacian_csr(ma_engine *e, int64_t n, const int32_t *rowptr,
+                                 const int32_t *col, const double *val, const double *g,
+                                 double *d, double rel_tol, int maxiter, double *res,
+                                 int64_t *iters) {
+  if (!e || n < 1 || !rowptr || !col || !val || !g || !d) return MA_BAD_INPUT;
+  if (n > INT32_MAX) return MA_BAD_INPUT;
+  std::vector<int32_t> hp(rowptr, rowptr + n + 1);
+  const int64_t nnz = hp[n];
+  if (hp[0] != 0 || nnz < 0) {
+    e->err = "row pointer must start at 0";
+    return MA_BAD_INPUT;
+  }
+  for (int64_t i = 0; i < n; i++)
+    if (hp[i + 1] < hp[i]) {
+      e->err = "row pointer must be non-decreasing";
+      return MA_BAD_INPUT;
+    }
+  for (int64_t q = 0; q < nnz; q++)
+    if (col[q] < 0 || col[q] >= n) {
+      e->err = "column index out of range";
+      return MA_BAD_INPUT;
+    }
+  SolverBuffers &S = e->sb;
+  MA_HIP_TRY(S.hptr.ensure((size_t)n + 1));
+  MA_HIP_TRY(S.hcol.ensure((size_t)nnz + 1));
+  MA_HIP_TRY(S.hval.ensure((size_t)nnz + 1));
+  MA_HIP_TRY(S.hg.ensure((size_t)n));
+  MA_HIP_TRY(S.hd.ensure((size_t)n));
+  MA_HIP_TRY(hipMemcpyAsync(S.hptr.p, rowptr, sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice,
+                            e->stream));
+  MA_HIP_TRY(hipMemcpyAsync(S.hcol.p, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice,
+                            e->stream));
+  MA_HIP_TRY(hipMemcpyAsync(S.hval.p, val, sizeof(double) * nnz, hipMemcpyHostToDevice,
+                            e->stream));
+  MA_HIP_TRY(hipMemcpyAsync(S.hg.p, g, sizeof(double) * n, hipMemcpyHostToDevice, e->stream));
+  CsrView M;
+  M.n = n;
+  M.nnz = nnz;
+  M.ptr = S.hptr.p;
+  M.col = S.hcol.p;
+  M.val = S.hval.p;
+  int64_t it = 0;
+  double rr = 0.0;
+  MA_HIP_TRY(pcg(e, M, (int32_t)(n - 1), S.hg.p, S.hd.p, rel_tol > 0 ? rel_tol : 1e-12,
+                 maxiter > 0 ? maxiter : 100000, 0, &it, &rr));
+  MA_HIP_TRY(hipMemcpyAsync(d, S.hd.p, sizeof(double) * n, hipMemcpyDeviceToHost, e->stream));
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
   if (res) *res = rr;
   if (iters) *iters = it;
   return MA_OK;
@@ -676,11 +747,21 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
   }
   ma_stats st{};
   double ms_eval = 0.0, ms_lin = 0.0;
+  // Mixed precision: evaluations run with FP32 geometry while the max
+  // relative mass error is above kSwitch (FP32 masses are good to about
+  // 1e-6 relative), then in FP64.  The damping test, the CG scalars and the
+  // stopping test are FP64 throughout, and the solve never stops on an FP32
+  // evaluation: the final check is always an FP64 evaluation.
+  constexpr double kSwitch = 1e-4, kStall = 0.05;
+  bool fp32_phase = o.precision == MA_MIXED;
+  bool last_fp64 = false;
   auto eval_at = [&](const double *wm, double &fval, NewtonStats &ns) -> ma_status {
     auto t0 = std::chrono::steady_clock::now();
     hipMemcpyAsync(B.w_s.p, wm, sizeof(double) * n, hipMemcpyDeviceToDevice, e->stream);
-    ma_status r = eval_morton(e, o.precision, &fval);
+    ma_status r = eval_morton(e, fp32_phase ? MA_MIXED : MA_FP64, &fval);
+    last_fp64 = !fp32_phase;
     st.neval++;
+    if (fp32_phase) st.neval_fp32++;
     if (r != MA_OK) return r;
     hipError_t he = newton_stats(e, B.mass_s.p, S.nu.p, wm, S.g.p, ns);
     if (he != hipSuccess) {
@@ -714,7 +795,15 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
   };
   int64_t niter = 0;
   const double l2_0 = ns.l2;
-  while (result == MA_OK && keep_going(ns)) {
+  while (result == MA_OK) {
+    if (!keep_going(ns)) {
+      if (last_fp64) break;
+      // converged on an FP32 evaluation: confirm (or continue) in FP64
+      fp32_phase = false;
+      s = eval_at(S.w.p, fval, ns);
+      if (s != MA_OK) return s;
+      continue;
+    }
     if (niter++ > o.maxiter) {  // `niter++ <= maxiter` (optimal_transport.hpp:151)
       result = MA_NOT_CONVERGED;
       break;
@@ -728,7 +817,10 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
     }
     int64_t it = 0;
     double rr = 0.0;
-    MA_HIP_TRY(pcg(e, S.g.p, S.d.p, rel, o.cg_maxiter, o.preconditioner, &it, &rr));
+    CsrView M;
+    int32_t gi = 0;
+    MA_HIP_TRY(eval_csr(e, M, gi));
+    MA_HIP_TRY(pcg(e, M, gi, S.g.p, S.d.p, rel, o.cg_maxiter, o.preconditioner, &it, &rr));
     hipLaunchKernelGGL(k_neg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, n, S.d.p);
     st.cg_iter += it;
     MA_HIP_TRY(hipStreamSynchronize(e->stream));
@@ -753,6 +845,12 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
     }
     if (result != MA_OK) break;
     std::swap(S.w.p, S.wtrial.p);
+    // leave the FP32 phase at kSwitch, or earlier when FP32 progress stalls
+    // close to its accuracy floor (a damped step, or less than a halving of
+    // the error, once below kStall)
+    if (fp32_phase && (ns.maxrel <= kSwitch ||
+                       (ns0.maxrel < kStall && (alpha < 1.0 || ns.maxrel > 0.5 * ns0.maxrel))))
+      fp32_phase = false;
     if (o.verbose)
       fprintf(stderr, "it %lld: f=%.17g |df|=%.6g maxrel=%.3g tau=%g eval=%lld cg=%lld\n",
               (long long)niter, fval - ns.nuw, ns.l2, ns.maxrel, alpha, (long long)st.neval,
@@ -790,6 +888,9 @@ ma_status ma_set_option(ma_engine *e, int option, int64_t value) {
       if (value < 4 || value > 4096) return MA_BAD_INPUT;
       e->cap_cand = (int)value;
       return MA_OK;
+    case MA_OPT_GRAPHS:
+      e->use_graphs = value != 0;
+      return MA_OK;
     default:
       return MA_BAD_INPUT;
   }
@@ -804,10 +905,44 @@ ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info) {
 }  // extern "C"
 
 namespace ma {
+
+// dPhi/dy_k = int_{Lag_k} rho 2 (y_k - x) dx = -2 (first moment about y_k),
+// written in the caller's order (interleaved x, y).
+__global__ void k_adjoint_dy(int64_t n, const int32_t *perm, const double *mom_local,
+                             double *dy) {
+  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = perm[k];
+  dy[2 * i] = -2.0 * mom_local[2 * k];
+  dy[2 * i + 1] = -2.0 * mom_local[2 * k + 1];
+}
+
+// Partial derivatives of the dual value Phi(Y, w, rho) = int rho(x) min_i
+// (|x - y_i|^2 - w_i) dx at fixed weights.  At the optimal weights they are
+// the derivatives of W2^2 (envelope theorem): the centroid form 2 m_i (y_i -
+// b_i) of include/MA/lloyd.hpp:126-144 for the positions, and
+// sum_i int_{Lag_i} (|x - y_i|^2 - w_i) phi_v(x) dx for the vertex values.
 ma_status adjoint(ma_engine *e, double *dY, double *dval) {
-  (void)dY;
-  (void)dval;
-  e->err = "adjoint not available";
-  return MA_NO_DATA;
+  EvalBuffers &B = e->eb;
+  MA_HIP_TRY(B.dval.ensure((size_t)e->nv));
+  MA_HIP_TRY(hipMemsetAsync(B.dval.p, 0, sizeof(double) * e->nv, e->stream));
+  ma_status s = evaluate(e, 2, nullptr);
+  if (s != MA_OK) return s;
+  if (dY) {
+    DBuf<double> tmp;
+    MA_HIP_TRY(tmp.ensure((size_t)(2 * e->n)));
+    hipLaunchKernelGGL(k_adjoint_dy, dim3((unsigned)((e->n + 127) / 128)), dim3(128), 0,
+                       e->stream, e->n, e->perm.p, B.mom_s.p, tmp.p);
+    MA_HIP_TRY(hipMemcpyAsync(dY, tmp.p, sizeof(double) * 2 * e->n, hipMemcpyDeviceToHost,
+                              e->stream));
+    MA_HIP_TRY(hipStreamSynchronize(e->stream));
+    tmp.release();
+  }
+  if (dval) {
+    MA_HIP_TRY(hipMemcpyAsync(dval, B.dval.p, sizeof(double) * e->nv, hipMemcpyDeviceToHost,
+                              e->stream));
+    MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  }
+  return MA_OK;
 }
 }  // namespace ma

@@ -46,15 +46,22 @@ __device__ inline void tri_bbox(const TriData &t, double &x0, double &y0, double
   y1 = fmax(t.y[0], fmax(t.y[1], t.y[2]));
 }
 
+// Offsets of the levels of the max-weight pyramid (level 0 = bins).
+struct PyrOff {
+  int64_t o[16];
+};
+
 // Albrecht-Collatz 6-point rule points (include/MA/quadrature.hpp:23-42):
 // weights 1/30 for points 0..2, 9/30 for points 3..5, times the area.
-struct AC6 {
-  double px[6], py[6];
+template <class R>
+struct AC6T {
+  R px[6], py[6];
 };
-__device__ inline void ac6_points(double ax, double ay, double bx, double by, double cx,
-                                  double cy, AC6 &q) {
-  const double ux = bx - ax, uy = by - ay, vx = cx - ax, vy = cy - ay;
-  const double h = 0.5, s6 = 1.0 / 6.0, t3 = 2.0 / 3.0;
+using AC6 = AC6T<double>;
+template <class R>
+__device__ inline void ac6_points(R ax, R ay, R bx, R by, R cx, R cy, AC6T<R> &q) {
+  const R ux = bx - ax, uy = by - ay, vx = cx - ax, vy = cy - ay;
+  const R h = R(0.5), s6 = R(1.0 / 6.0), t3 = R(2.0 / 3.0);
   q.px[0] = ax + h * ux + h * vx;
   q.py[0] = ay + h * uy + h * vy;
   q.px[1] = ax + h * ux;

@@ -298,10 +298,6 @@ __global__ void k_pyr_up(const double *lo, double *hi, int64_t nhi) {
   hi[m] = fmax(fmax(lo[4 * m], lo[4 * m + 1]), fmax(lo[4 * m + 2], lo[4 * m + 3]));
 }
 
-struct PyrOff {
-  int64_t o[16];
-};
-
 // ---------------------------------------------------------------------------
 // Candidate owners of a bin.  A Dirac j can own a point of bin b only if
 //   min_{x in b} pi_j(x) <= U_b := max_{x in b} pi_k(x)   for any competitor k,
@@ -605,7 +601,9 @@ __device__ inline bool clip_tri_edge(const Poly &P, Poly &Q, double ax, double a
 // General per-Dirac kernel (polygons up to kMaxPoly vertices in private
 // memory) for the cells listed in `list`: the rare cells that overflow the
 // register/LDS fast path.
-template <int MODE>  // 0: Kantorovich, 1: first moments
+// MODE 0: Kantorovich, 1: first moments, 2: first moments + density-value
+// adjoint (integrals of (|x - y_k|^2 - w_k) phi_v, accumulated into dval).
+template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_cells_general(
     const int32_t *__restrict__ list, const int32_t *__restrict__ nlist,
     Grid g, int64_t n, const double *__restrict__ pos, const double *__restrict__ w,
@@ -614,7 +612,8 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
     const int32_t *__restrict__ tbin_start, const int32_t *__restrict__ tbin_items,
     const TriData *__restrict__ tri, double *mass_out, double *intg_out, double *fval_part,
     double *mom_out, int32_t *row_cnt, int32_t *row_col, double *row_val, int cap_row,
-    int32_t *flags, unsigned long long *counters) {
+    const int32_t *__restrict__ triv, double *dval, int32_t *flags,
+    unsigned long long *counters) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= *nlist) return;
   const int64_t k = list[idx];
@@ -731,6 +730,13 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
           }
           nfrag++;
           const double r0 = t.a * yx + t.b * yy + t.c;  // rho at the Dirac
+          double la[3], lb[3], lc[3], dv[3] = {0.0, 0.0, 0.0};
+          if (MODE == 2) {
+            const double inv = 1.0 / ((bxx - ax) * (cy - ay) - (cx - ax) * (byy - ay));
+            la[0] = (byy - cy) * inv; lb[0] = (cx - bxx) * inv; lc[0] = (bxx * cy - cx * byy) * inv;
+            la[1] = (cy - ay) * inv;  lb[1] = (ax - cx) * inv;  lc[1] = (cx * ay - ax * cy) * inv;
+            la[2] = (ay - byy) * inv; lb[2] = (bxx - ax) * inv; lc[2] = (ax * byy - bxx * ay) * inv;
+          }
           // fan from vertex 0 (include/MA/quadrature.hpp:122-143)
           for (int i = 1; i + 1 < m; i++) {
             const double x0 = Q.x[0], y0 = Q.y[0], x1 = Q.x[i], y1 = Q.y[i];
@@ -741,9 +747,18 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
             AC6 qp;
             ac6_points(x0, y0, x1, y1, x2, y2, qp);
             double s1 = 0.0, s9 = 0.0, m1x = 0.0, m9x = 0.0, m1y = 0.0, m9y = 0.0;
+            double c1[3] = {0.0, 0.0, 0.0}, c9[3] = {0.0, 0.0, 0.0};
 #pragma unroll
             for (int p = 0; p < 6; p++) {
               const double rp = r0 + t.a * qp.px[p] + t.b * qp.py[p];
+              if (MODE == 2) {
+                const double c = qp.px[p] * qp.px[p] + qp.py[p] * qp.py[p] - wk;
+#pragma unroll
+                for (int v = 0; v < 3; v++) {
+                  const double f = c * (la[v] * qp.px[p] + lb[v] * qp.py[p] + lc[v]);
+                  if (p < 3) c1[v] += f; else c9[v] += f;
+                }
+              }
               if (MODE == 0) {
                 const double f = rp * (qp.px[p] * qp.px[p] + qp.py[p] * qp.py[p]);
                 if (p < 3) s1 += f; else s9 += f;
@@ -763,6 +778,16 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
               momx += area * (m1x * (1.0 / 30.0) + m9x * (9.0 / 30.0));
               momy += area * (m1y * (1.0 / 30.0) + m9y * (9.0 / 30.0));
             }
+            if (MODE == 2) {
+#pragma unroll
+              for (int v = 0; v < 3; v++)
+                dv[v] += area * (c1[v] * (1.0 / 30.0) + c9[v] * (9.0 / 30.0));
+            }
+          }
+          if (MODE == 2) {
+            const int64_t ti = tbin_items[q];
+#pragma unroll
+            for (int v = 0; v < 3; v++) atomicAdd(&dval[triv[3 * ti + v]], dv[v]);
           }
           if (MODE == 0) {
             // Hessian contributions of the fragment's cell edges
@@ -832,13 +857,13 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
     row_cnt[k] = cnt;
   } else {
     mass_out[k] = mass;
-    mom_out[2 * k] = momx + yx * mass;
-    mom_out[2 * k + 1] = momy + yy * mass;
+    // mode 2 keeps the moments about the Dirac
+    mom_out[2 * k] = momx + (MODE == 1 ? yx * mass : 0.0);
+    mom_out[2 * k + 1] = momy + (MODE == 1 ? yy * mass : 0.0);
   }
   if (overflow) atomicOr(&flags[0], 1);
   atomicAdd(&counters[0], (unsigned long long)nfrag);
   atomicMax(&counters[2], (unsigned long long)np);
-  atomicAdd(&counters[4], 1ull);
 }
 
 __global__ void k_csr_fill(int64_t n, const int32_t *row_cnt, const int32_t *row_col,
@@ -886,7 +911,7 @@ hipError_t device_sum(ma_engine *e, const double *x, int64_t n, double *d_out) {
 // ---------------------------------------------------------------------------
 // One evaluation.
 // ---------------------------------------------------------------------------
-ma_status run_cells(ma_engine *e, int mode, int32_t *npairs_out, int32_t *ngeneral_out);
+ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out, int32_t *ngeneral_out);
 
 // Candidate owners of every domain bin and the bin rectangles of the Diracs
 // marked pn == -2 (see k_candidates, k_rect).
@@ -919,7 +944,7 @@ ma_status run_bin_candidates(ma_engine *e, const int32_t *pn) {
   return MA_OK;
 }
 
-ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
+ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   const Grid &g = e->grid;
   const int64_t n = e->n;
   EvalBuffers &B = e->eb;
@@ -964,25 +989,18 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
   MA_HIP_TRY(hipMemsetAsync(B.counters.p, 0, sizeof(unsigned long long) * 8, e->stream));
   MA_HIP_TRY(hipEventRecord(e->ev[2], e->stream));
   int32_t npairs = 0, ngen = 0;
-  ma_status cs = run_cells(e, mode, &npairs, &ngen);
+  ma_status cs = run_cells(e, mode, fp32, &npairs, &ngen);
   if (cs != MA_OK) return cs;
   if (ngen > 0) {
     // cells that overflowed the fast path
-    if (mode == 0) {
-      hipLaunchKernelGGL(k_cells_general<0>, dim3(nblocks(ngen)), dim3(kBlock), 0, e->stream,
-                         B.glist.p, B.glist.p + n, g, n, e->pos.p, B.w_s.p, e->perm.p, B.rect.p,
-                         B.cand_cnt.p, B.cand.p, e->cap_cand, e->tbin_start.p, e->tbin_items.p,
-                         e->tri.p, B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p,
-                         B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row, B.flags.p,
-                         B.counters.p);
-    } else {
-      hipLaunchKernelGGL(k_cells_general<1>, dim3(nblocks(ngen)), dim3(kBlock), 0, e->stream,
-                         B.glist.p, B.glist.p + n, g, n, e->pos.p, B.w_s.p, e->perm.p, B.rect.p,
-                         B.cand_cnt.p, B.cand.p, e->cap_cand, e->tbin_start.p, e->tbin_items.p,
-                         e->tri.p, B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p,
-                         B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row, B.flags.p,
-                         B.counters.p);
-    }
+    auto kern = mode == 0 ? k_cells_general<0> : mode == 1 ? k_cells_general<1>
+                                                            : k_cells_general<2>;
+    hipLaunchKernelGGL(kern, dim3(nblocks(ngen)), dim3(kBlock), 0, e->stream, B.glist.p,
+                       B.glist.p + n, g, n, e->pos.p, B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p,
+                       B.cand.p, e->cap_cand, e->tbin_start.p, e->tbin_items.p, e->tri.p,
+                       B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p, B.row_cnt.p,
+                       B.row_col.p, B.row_val.p, e->cap_row, e->triv.p, B.dval.p, B.flags.p,
+                       B.counters.p);
   }
   MA_HIP_TRY(hipEventRecord(e->ev[6], e->stream));
   if (mode == 0) {
@@ -1016,7 +1034,7 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
   MA_HIP_TRY(hipStreamSynchronize(e->stream));
   float t[7] = {0, 0, 0, 0, 0, 0, 0}, tc = 0.0f;
   for (int i = 0; i < 7; i++) MA_HIP_TRY(hipEventElapsedTime(&t[i], e->ev[i], e->ev[i + 1]));
-  if (e->info.nbinpath > 0 || ngen > 0) MA_HIP_TRY(hipEventElapsedTime(&tc, e->ev[8], e->ev[9]));
+  if (e->ran_candidates) MA_HIP_TRY(hipEventElapsedTime(&tc, e->ev[8], e->ev[9]));
   e->info.ms_weights = t[0];
   e->info.ms_candidates = tc;
   e->info.ms_cell_build = t[2];
@@ -1029,6 +1047,9 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out) {
   e->info.ncand = (int64_t)ct[1];
   e->info.max_poly = (int64_t)ct[2];
   e->info.empty_cells = (int64_t)ct[3];
+  e->info.nvert = (int64_t)ct[4];
+  e->info.nredecide = (int64_t)ct[5];
+  e->info.nbincand = (int64_t)ct[6];
   e->info.npairs = npairs;
   e->info.ngeneral = ngen;
   e->info.nnz = e->nnz;

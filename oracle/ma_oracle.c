@@ -708,10 +708,16 @@ static int ti_clip(const tictx *c, const pedge *P, int n, int64_t wv, pedge *R) 
 }
 
 /* edge_to_line (voronoi_triangulation_intersection.hpp:98-110), as
- * a x + b y + c = 0 with CGAL's Line_2(p,q) and radical_axis conventions. */
-static void edge_line(const tictx *c, const pedge *e, double *l) {
+ * a x + b y + c = 0 with CGAL's Line_2(p,q) and radical_axis conventions,
+ * in coordinates relative to the origin (ox, oy).  The reference constructs
+ * these lines in absolute coordinates (inexact constructions), where the
+ * radical axis of two nearly coincident Diracs loses accuracy by
+ * cancellation (|P|^2 - |Q|^2 with |P - Q| << |P|); relative to the Dirac
+ * whose cell is being cut the same construction is accurate. */
+static void edge_line(const tictx *c, const pedge *e, double ox, double oy, double *l) {
   if (e->type == EDGE_T) {
-    double px = c->t->x[e->a], py = c->t->y[e->a], qx = c->t->x[e->b], qy = c->t->y[e->b];
+    double px = c->t->x[e->a] - ox, py = c->t->y[e->a] - oy;
+    double qx = c->t->x[e->b] - ox, qy = c->t->y[e->b] - oy;
     l[0] = py - qy;
     l[1] = qx - px;
     l[2] = px * qy - py * qx;
@@ -719,20 +725,24 @@ static void edge_line(const tictx *c, const pedge *e, double *l) {
     double P[3], Q[3];
     wp(c, c->v, P);
     wp(c, e->b, Q);
+    P[0] -= ox; P[1] -= oy;
+    Q[0] -= ox; Q[1] -= oy;
     l[0] = 2.0 * (P[0] - Q[0]);
     l[1] = 2.0 * (P[1] - Q[1]);
     l[2] = -(P[0] * P[0] + P[1] * P[1]) + P[2] + (Q[0] * Q[0] + Q[1] * Q[1]) - Q[2];
   }
 }
-/* vertex_to_point / line_line_intersection (predicates.hpp:21-30) */
+/* vertex_to_point / line_line_intersection (predicates.hpp:21-30), formed
+ * relative to the Dirac v and translated back. */
 static void vertex_point(const tictx *c, const pedge *e1, const pedge *e2, double *x,
                          double *y) {
+  const double ox = c->X[2 * c->v], oy = c->X[2 * c->v + 1];
   double l1[3], l2[3];
-  edge_line(c, e1, l1);
-  edge_line(c, e2, l2);
+  edge_line(c, e1, ox, oy, l1);
+  edge_line(c, e2, ox, oy, l2);
   double den = l1[0] * l2[1] - l2[0] * l1[1];
-  *x = (l1[1] * l2[2] - l1[2] * l2[1]) / den;
-  *y = (l1[2] * l2[0] - l1[0] * l2[2]) / den;
+  *x = ox + (l1[1] * l2[2] - l1[2] * l2[1]) / den;
+  *y = oy + (l1[2] * l2[0] - l1[0] * l2[2]) / den;
 }
 
 /* ====================================================================== */

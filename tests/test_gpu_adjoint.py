@@ -1,0 +1,167 @@
+"""Adjoint (dW2^2/dY, dW2^2/d vertex density values) through the C ABI.
+
+Checks, in order of strength:
+* position part against the exact first moments of the golden fixtures and
+  against the oracle's moments (the centroid form of include/MA/lloyd.hpp);
+* density part against the exact linearity of the dual value in rho at fixed
+  weights: Phi(rho + d) - Phi(rho) = <d, dval> (oracle values of Phi);
+* total derivatives of W2^2 (re-solved at every perturbation) against central
+  finite differences, h = 1e-6, to 1e-6 relative;
+* the torch.autograd.Function wrapper and gradient-driven Lloyd steps against
+  the oracle's centroid update.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from mongeampere_amd import configs
+from mongeampere_amd.configs import Mesh
+from conftest import GOLDEN_CASES, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _phi(mesh, X, w):
+    """Dual value Phi = int rho min_i(|x - y_i|^2 - w_i) = -fval (oracle)."""
+    f, _, _ = oracle.kantorovich(mesh, X, w)
+    return -f
+
+
+@pytest.mark.parametrize("name", GOLDEN_CASES)
+def test_adjoint_dy_golden(engine, name):
+    """dY = 2 (m y - M) with the exact masses and moments of the fixture."""
+    c = load_golden(name)
+    engine.set_mesh(c["mesh"]).set_diracs(c["X"])
+    dY, dval = engine.adjoint(c["w"])
+    ref = 2.0 * (c["mass"][:, None] * c["X"] - c["mom"])
+    scale = max(np.max(np.abs(ref)), np.max(c["mass"]) * 1e-3)
+    assert np.max(np.abs(dY - ref)) <= 1e-10 * scale
+    assert dval.shape == (c["mesh"].vxy.shape[0],)
+
+
+def _random_weights(p, seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    N = p.X.shape[0]
+    return rng.uniform(-scale, scale, N) / N
+
+
+def test_adjoint_dy_vs_oracle_moments(engine):
+    p = configs.config1()
+    w = _random_weights(p, 11)
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    dY, _ = engine.adjoint(w)
+    m, mom = oracle.first_moment(p.mesh, p.X, w)
+    ref = 2.0 * (m[:, None] * p.X - mom)
+    assert np.max(np.abs(dY - ref)) <= 1e-10 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("force_general", [0, 1])
+def test_adjoint_dval_linearity(engine, force_general):
+    """Phi is linear in rho at fixed (Y, w): the difference quotient is exact."""
+    p = configs.config1(N=400, n=24)
+    w = _random_weights(p, 12)
+    rng = np.random.default_rng(13)
+    engine.set_option("force_general", force_general)
+    try:
+        engine.set_mesh(p.mesh).set_diracs(p.X)
+        _, dval = engine.adjoint(w)
+    finally:
+        engine.set_option("force_general", 0)
+    base = _phi(p.mesh, p.X, w)
+    V = p.mesh.vxy.shape[0]
+    for trial in range(3):
+        d = rng.standard_normal(V)
+        m2 = Mesh(p.mesh.vxy, p.mesh.vval + d, p.mesh.tri)
+        diff = _phi(m2, p.X, w) - base
+        assert abs(diff - d @ dval) <= 1e-10 * (abs(diff) + np.abs(d) @ np.abs(dval))
+    for v in rng.choice(V, 4, replace=False):
+        d = np.zeros(V)
+        d[v] = 1.0
+        m2 = Mesh(p.mesh.vxy, p.mesh.vval + d, p.mesh.tri)
+        diff = _phi(m2, p.X, w) - base
+        assert abs(diff - dval[v]) <= 1e-10 * max(abs(dval[v]), 1e-3 * np.max(np.abs(dval)))
+
+
+def _vertex_mass(mesh):
+    """int phi_v over the mesh (a third of the adjacent triangle areas)."""
+    a = mesh.vxy[mesh.tri]
+    ar = 0.5 * np.abs((a[:, 1, 0] - a[:, 0, 0]) * (a[:, 2, 1] - a[:, 0, 1])
+                      - (a[:, 2, 0] - a[:, 0, 0]) * (a[:, 1, 1] - a[:, 0, 1]))
+    c = np.zeros(mesh.vxy.shape[0])
+    for k in range(3):
+        np.add.at(c, mesh.tri[:, k], ar / 3.0)
+    return c
+
+
+def _w2(engine, mesh, X, nu, w0):
+    engine.set_mesh(mesh).set_diracs(X)
+    w, st = engine.ot_solve(nu, x=w0, stop="maxrel", tol=1e-11, maxiter=60)
+    assert st.status == "MA_OK"
+    return -st.fval, w
+
+
+def test_adjoint_total_derivative_fd(engine):
+    """Central differences of W2^2 (solved to max-rel 1e-11 at every point),
+    h = 1e-6, on random coordinates and mass-preserving vertex directions."""
+    p = configs.config4(N=2000, n=48)
+    W, w = _w2(engine, p.mesh, p.X, p.nu, None)
+    dY, dval = engine.adjoint(w)
+    rng = np.random.default_rng(14)
+    h = 1e-6
+    for i in rng.choice(p.X.shape[0], 8, replace=False):
+        for c in (0, 1):
+            Xp, Xm = p.X.copy(), p.X.copy()
+            Xp[i, c] += h
+            Xm[i, c] -= h
+            fd = (_w2(engine, p.mesh, Xp, p.nu, w)[0] - _w2(engine, p.mesh, Xm, p.nu, w)[0]) / (2 * h)
+            assert abs(fd - dY[i, c]) <= 1e-6 * abs(dY[i, c]) + 1e-9 * np.max(np.abs(dY)), (i, c)
+    cv = _vertex_mass(p.mesh)
+    V = p.mesh.vxy.shape[0]
+    for v in rng.choice(V, 8, replace=False):
+        u = (v + V // 2) % V
+        d = np.zeros(V)
+        d[v] = 1.0
+        d[u] = -cv[v] / cv[u]  # keeps int rho = sum(nu)
+        mp = Mesh(p.mesh.vxy, p.mesh.vval + h * d, p.mesh.tri)
+        mm = Mesh(p.mesh.vxy, p.mesh.vval - h * d, p.mesh.tri)
+        fd = (_w2(engine, mp, p.X, p.nu, w)[0] - _w2(engine, mm, p.X, p.nu, w)[0]) / (2 * h)
+        an = d @ dval
+        assert abs(fd - an) <= 1e-6 * abs(an) + 1e-9 * np.abs(d) @ np.abs(dval), v
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+
+
+def test_autograd_function(engine):
+    import torch
+    from mongeampere_amd.autograd import w2_squared
+
+    p = configs.config4(N=1500, n=40)
+    Y = torch.tensor(p.X, dtype=torch.float64, requires_grad=True)
+    vv = torch.tensor(p.mesh.vval, dtype=torch.float64, requires_grad=True)
+    W = w2_squared(Y, vv, engine, p.mesh, p.nu, tol=1e-11)
+    (3.0 * W).backward()
+    w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-11)
+    dY, dval = engine.adjoint(w)
+    assert W.item() > 0.0
+    assert abs(W.item() + st.fval) <= 1e-12 * W.item()
+    assert np.max(np.abs(Y.grad.numpy() - 3.0 * dY)) <= 1e-8 * np.max(np.abs(dY))
+    assert np.max(np.abs(vv.grad.numpy() - 3.0 * dval)) <= 1e-8 * np.max(np.abs(dval))
+
+
+def test_lloyd_steps_from_gradient(engine):
+    """Optimal-transport Lloyd iteration driven by the GPU gradient,
+    y <- y - dW2^2/dy / (2 m), equals the oracle's centroid update at the
+    same weights (include/MA/lloyd.hpp:126-144) at every step."""
+    p = configs.config4(N=1500, n=40)
+    X = p.X.copy()
+    for step in range(5):
+        # each step restarts from w = 0 (the moved Diracs can leave a cell
+        # empty at the previous weights)
+        engine.set_mesh(p.mesh).set_diracs(X)
+        w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-10)
+        assert st.status == "MA_OK"
+        dY, _ = engine.adjoint(w)
+        m, _ = engine.first_moment(w)
+        Xn = X - dY / (2.0 * m[:, None])
+        _, b = oracle.lloyd(p.mesh, X, w)
+        assert np.max(np.abs(Xn - b)) <= 1e-11, step
+        X = Xn

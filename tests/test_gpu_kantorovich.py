@@ -97,6 +97,24 @@ def test_config1_vs_oracle(engine, wscale):
     assert _pattern(H, 1e-9) == _pattern(oH, 1e-9)
 
 
+@pytest.mark.parametrize("stage", [1, 3])
+def test_newton_weights_vs_oracle(engine, stage):
+    """Weights from the first Newton steps of a C3-like problem: large weight
+    spread, so many cells are finished by the max-weight pyramid walk."""
+    p = configs.config3(N=20_000, n=128)
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-12, maxiter=stage - 1,
+                            raise_on_error=False)
+    fval, g, H = engine.kantorovich(w)
+    info = engine.eval_info()
+    print(f"stage {stage}: pyramid-walk cells {info['nbinpath']}", flush=True)
+    of, og, oH = oracle.kantorovich(p.mesh, p.X, w)
+    assert _rel(g, og) <= TOL
+    assert abs(fval - of) <= TOL * abs(of)
+    D = (H - oH)
+    assert abs(D).max() <= TOL * abs(oH).max()
+
+
 @pytest.mark.slow
 def test_config2_vs_oracle(engine):
     p = configs.config2()

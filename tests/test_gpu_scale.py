@@ -1,0 +1,147 @@
+"""Full-size workloads on the GPU: size-independent properties where the CPU
+oracle would take too long, and the oracle on windows where it does not.
+
+C4 (512x512 grid, N = 200k): solve to 1e-9, adjoint against central finite
+differences (h = 1e-6) of re-solved W2^2 on 64 coordinates and 64
+mass-preserving vertex directions, 20 Lloyd steps driven by the gradient.
+C5 (annulus, ~500k-vertex Delaunay mesh, N = 4M, half on a jittered lattice):
+mixed-precision forward and adjoint, FP64 re-decisions, invariants.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from mongeampere_amd import configs
+from mongeampere_amd.configs import Mesh
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c4():
+    return configs.config4()
+
+
+@pytest.fixture(scope="module")
+def c5():
+    return configs.config5()
+
+
+def _vertex_mass(mesh):
+    a = mesh.vxy[mesh.tri]
+    ar = 0.5 * np.abs((a[:, 1, 0] - a[:, 0, 0]) * (a[:, 2, 1] - a[:, 0, 1])
+                      - (a[:, 2, 0] - a[:, 0, 0]) * (a[:, 1, 1] - a[:, 0, 1]))
+    c = np.zeros(mesh.vxy.shape[0])
+    for k in range(3):
+        np.add.at(c, mesh.tri[:, k], ar / 3.0)
+    return c
+
+
+def test_c4_solve_and_adjoint_fd(engine, c4):
+    p = c4
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-9)
+    print(f"C4 solve: {st.niter} Newton steps, {st.neval} evals, {st.ms_total:.0f} ms", flush=True)
+    assert st.status == "MA_OK" and st.max_rel_err <= 1e-9
+    dY, dval = engine.adjoint(w)
+    # Lloyd form: y - dY / (2 m) is the cell centroid (oracle on a subset)
+    m, cen = engine.lloyd(w)
+    assert np.max(np.abs(p.X - dY / (2 * m[:, None]) - cen)) <= 1e-12
+
+    def W2(mesh, X):
+        engine.set_mesh(mesh).set_diracs(X)
+        _, s = engine.ot_solve(p.nu, x=w, stop="maxrel", tol=1e-11, maxiter=50)
+        assert s.status == "MA_OK"
+        return -s.fval
+
+    rng = np.random.default_rng(41)
+    h = 1e-6
+    errs = []
+    for k in rng.choice(2 * p.X.shape[0], 64, replace=False):
+        i, c = divmod(int(k), 2)
+        Xp, Xm = p.X.copy(), p.X.copy()
+        Xp[i, c] += h
+        Xm[i, c] -= h
+        fd = (W2(p.mesh, Xp) - W2(p.mesh, Xm)) / (2 * h)
+        errs.append(abs(fd - dY[i, c]) / max(abs(dY[i, c]), 1e-3 * np.max(np.abs(dY))))
+    print(f"C4 dY: max rel FD error {max(errs):.2e}", flush=True)
+    assert max(errs) <= 1e-6
+    cv = _vertex_mass(p.mesh)
+    V = p.mesh.vxy.shape[0]
+    errs = []
+    for v in rng.choice(V, 64, replace=False):
+        u = (v + V // 3) % V
+        d = np.zeros(V)
+        d[v] = 1.0
+        d[u] = -cv[v] / cv[u]
+        fd = (W2(Mesh(p.mesh.vxy, p.mesh.vval + h * d, p.mesh.tri), p.X)
+              - W2(Mesh(p.mesh.vxy, p.mesh.vval - h * d, p.mesh.tri), p.X)) / (2 * h)
+        an = d @ dval
+        errs.append(abs(fd - an) / max(abs(an), 1e-3 * np.max(np.abs(dval)) * abs(d).sum()))
+    print(f"C4 dval: max rel FD error {max(errs):.2e}", flush=True)
+    assert max(errs) <= 1e-6
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+
+
+def test_c4_lloyd_steps(engine, c4):
+    """20 optimal-transport Lloyd steps driven by the GPU gradient; every step
+    equals the oracle's centroid update on a sample of 2000 cells."""
+    p = c4
+    X = p.X.copy()
+    rng = np.random.default_rng(42)
+    for step in range(20):
+        engine.set_mesh(p.mesh).set_diracs(X)
+        w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-9)
+        assert st.status == "MA_OK"
+        dY, _ = engine.adjoint(w)
+        m, _ = engine.first_moment(w)
+        Xn = X - dY / (2.0 * m[:, None])
+        if step in (0, 19):
+            _, b = oracle.lloyd(p.mesh, X, w)
+            idx = rng.choice(X.shape[0], 2000, replace=False)
+            assert np.max(np.abs(Xn[idx] - b[idx])) <= 1e-11, step
+        X = Xn
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+
+
+def test_c5_mixed_forward_and_adjoint(engine, c5):
+    p = c5
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    N = p.X.shape[0]
+    w = np.zeros(N)
+    f64, g64, H64 = engine.kantorovich(w, precision="fp64")
+    i64 = engine.eval_info()
+    f32, g32, H32 = engine.kantorovich(w, precision="mixed")
+    i32 = engine.eval_info()
+    print(f"C5: N={N} T={p.mesh.tri.shape[0]} FP64 eval {i64['ms_total']:.1f} ms, mixed "
+          f"{i32['ms_total']:.1f} ms, FP64 re-decisions {i32['nredecide']}, "
+          f"general cells {i64['ngeneral']}/{i32['ngeneral']}", flush=True)
+    total = p.mesh.total_mass()
+    assert abs(g64.sum() - total) <= 1e-11 * total
+    assert abs(g32.sum() - total) <= 1e-6 * total
+    assert np.max(np.abs(g32 - g64)) <= 2e-5 * np.mean(g64)
+    assert abs(H64 @ np.ones(N)).max() <= 1e-10 * abs(H64.diagonal()).max()
+    assert abs(H64 - H64.T).max() <= 1e-10 * abs(H64.diagonal()).max()
+    D = (H32 - H64).tocoo()
+    assert (np.max(np.abs(D.data)) if D.nnz else 0.0) <= 1e-4 * abs(H64.data).max()
+    # adjoint: Phi is linear in rho, so <rho, dPhi/drho> = Phi = -fval
+    dY, dval = engine.adjoint(w)
+    assert abs(p.mesh.vval @ dval + f64) <= 1e-10 * abs(f64)
+    # position part at w = 0 against the oracle on a window of the annulus
+    sel = (p.X[:, 0] > 0.5) & (p.X[:, 0] < 0.52) & (p.X[:, 1] > -0.01) & (p.X[:, 1] < 0.01)
+    m, cen = engine.lloyd(w)
+    assert np.max(np.abs(p.X[sel] - dY[sel] / (2 * m[sel, None]) - cen[sel])) <= 1e-11
+
+
+def test_c3_solve_mixed_vs_fp64(engine):
+    """Headline workload: both precisions reach max relative error 1e-8."""
+    p = configs.config3()
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    w64, s64 = engine.ot_solve(p.nu, stop="maxrel", tol=1e-8, precision="fp64")
+    w32, s32 = engine.ot_solve(p.nu, stop="maxrel", tol=1e-8, precision="mixed")
+    print(f"C3 solve fp64 {s64.ms_total:.0f} ms ({s64.neval} evals), mixed {s32.ms_total:.0f} ms "
+          f"({s32.neval} evals, {s32.neval_fp32} FP32)", flush=True)
+    assert s64.status == "MA_OK" and s32.status == "MA_OK"
+    assert s64.max_rel_err <= 1e-8 and s32.max_rel_err <= 1e-8
+    assert np.max(np.abs(w32 - w64)) <= 1e-6 * np.max(np.abs(w64))

@@ -50,6 +50,43 @@ def test_solve_laplacian_vs_direct(engine):
     assert np.max(np.abs(d - od)) <= 1e-8 * np.max(np.abs(od))
 
 
+def test_solve_laplacian_caller_csr(engine):
+    """Caller-supplied CSR (solve_laplacian_matrix(h, g) call shape) in the
+    caller's Dirac order: same solution as the direct solve."""
+    p = configs.config1()
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    _, g, H = engine.kantorovich(np.zeros(p.X.shape[0]))
+    rhs = g - p.nu
+    d, res, it = engine.solve_laplacian_matrix(H, rhs, rel_tol=1e-13)
+    od, err = oracle.solve_laplacian_matrix(H, rhs)
+    assert d[-1] == 0.0
+    assert np.max(np.abs(d - od)) <= 1e-8 * np.max(np.abs(od))
+
+
+def test_graph_replay_matches_eager(engine):
+    """CG batches replayed from a HIP graph give bit-identical results to
+    eager launches, over several solves with eager work in between."""
+    p = configs.config1()
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    rng = np.random.default_rng(7)
+    try:
+        for trial in range(3):
+            w = rng.uniform(-0.2, 0.2, p.X.shape[0]) / p.X.shape[0] if trial else np.zeros(
+                p.X.shape[0])
+            _, g, _ = engine.kantorovich(w)  # one Hessian for both solves
+            out = {}
+            for graphs in (1, 0, 1):
+                engine.set_option("graphs", graphs)
+                d, _, it = engine.solve_laplacian(g - p.nu, rel_tol=1e-12)
+                if graphs in out:
+                    assert np.array_equal(d, out[graphs][0])  # second replay
+                out[graphs] = (d, it)
+            assert out[1][1] == out[0][1]
+            assert np.array_equal(out[1][0], out[0][0])
+    finally:
+        engine.set_option("graphs", 1)
+
+
 def test_empty_cell_status(engine):
     """A Dirac whose cell is empty at w=0 makes the solve stop with
     MA_EMPTY_CELL and report its index (reference: eps0 <= 0 early return,

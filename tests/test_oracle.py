@@ -328,3 +328,28 @@ def test_oracle_empty_cell():
     X[7] = X[3]
     w, st = oracle.ot_solve(p.mesh, X, p.nu)
     assert st.status == "empty_cell"
+
+
+# --------------------------------------------------------------------------
+# nearly coincident Diracs: the radical axis of a pair 1e-5 apart, formed
+# relative to the Dirac, stays exact to rounding (exact-rational brute force)
+# --------------------------------------------------------------------------
+def test_near_coincident_pair_exact():
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_golden as mg
+
+    rng = np.random.default_rng(77)
+    vxy, tri = configs.grid_mesh(5)
+    vval = rng.uniform(0.5, 1.5, vxy.shape[0])
+    X = rng.uniform(0.05, 0.95, size=(14, 2))
+    X[1] = X[0] + np.array([1.1e-5, 0.7e-5])
+    X[3] = X[2] + np.array([-0.4e-5, 1.3e-5])
+    w = np.zeros(len(X))
+    ex = mg.solve(vxy, vval, tri, X, w)
+    mesh = configs.Mesh(vxy, vval, tri.astype(np.int32))
+    f, g, H = oracle.kantorovich(mesh, X, w)
+    em = np.array(ex["mass"])
+    assert np.max(np.abs(g - em) / em) <= 1e-12
+    assert abs(f - ex["fval"]) <= 1e-12 * abs(ex["fval"])
