@@ -173,6 +173,12 @@ ma_status ma_adjoint(ma_engine *e, const double *w, double *dY, double *dval);
 
 ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info);
 
+/* Per-cell outcome of the last evaluation, for the Diracs ids[0..count-1]
+ * (caller indices): out[i] = number of vertices of the cell polygon built by
+ * the register/LDS path (0: empty cell), or -1 if the cell was evaluated by
+ * the general kernel. */
+ma_status ma_cell_paths(ma_engine *e, int64_t count, const int64_t *ids, int32_t *out);
+
 /* Engine options.  MA_OPT_FORCE_GENERAL (0/1): evaluate every cell with the
  * general kernel instead of the register/LDS fast path (for testing the
  * overflow path); MA_OPT_CAP_CAND: initial capacity of the per-bin candidate

@@ -61,6 +61,7 @@ SIGNATURES = [
     ("ma_first_moment", I32, [P, P, P, P]),
     ("ma_adjoint", I32, [P, P, P, P]),
     ("ma_get_eval_info", I32, [P, P]),
+    ("ma_cell_paths", I32, [P, I64, P, P]),
     ("ma_default_solve_opts", None, [P]),
     ("ma_set_option", I32, [P, I32, I64]),
 ]

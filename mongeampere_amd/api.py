@@ -149,6 +149,14 @@ class Engine:
         self._check(self._L.ma_get_eval_info(self._h, ctypes.byref(info)))
         return {k: getattr(info, k) for k, _ in _lib.EvalInfo._fields_}
 
+    def cell_paths(self, ids):
+        """Per-cell outcome of the last evaluation: polygon vertex count of
+        the register/LDS path (0 = empty), or -1 for the general kernel."""
+        ids = np.ascontiguousarray(ids, np.int64)
+        out = np.empty(ids.shape[0], np.int32)
+        self._check(self._L.ma_cell_paths(self._h, ids.shape[0], _ptr(ids), _ptr(out)))
+        return out
+
     # ---- linear solve / Newton -------------------------------------------
     def solve_laplacian(self, g, rel_tol=1e-12, maxiter=0):
         g = np.ascontiguousarray(g, np.float64)

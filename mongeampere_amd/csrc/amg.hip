@@ -127,33 +127,33 @@ __global__ void k_amg_smooth0(int64_t n, const double *dinv, const double *b, do
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) x[i] = kOmega * b[i] * dinv[i];
 }
-// y = x + omega (b - A x) / D
+// y = x + omega (b - A x) / D; rows are read by groups of kRowLanes lanes
+// (consecutive entries of consecutive rows: coalesced)
 __global__ void k_amg_smooth(int64_t n, const int32_t *ptr, const int32_t *col,
                              const double *val, const double *dinv, const double *b,
                              const double *x, double *y, const double *done) {
   if (*done != 0.0) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double ax = 0.0;
-  for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) ax += val[q] * x[col[q]];
-  y[i] = x[i] + kOmega * (b[i] - ax) * dinv[i];
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = t / kRowLanes;
+  const int lane = (int)(t % kRowLanes);
+  const double ax = row_dot(ptr, col, val, x, i, i < n, lane);
+  if (i < n && lane == 0) y[i] = x[i] + kOmega * (b[i] - ax) * dinv[i];
 }
-// bc[I] = sum over the aggregate of (b - A x)
+// bc[I] = sum over the aggregate of (b - A x): kAgg rows x kRowLanes lanes
 __global__ void k_amg_restrict(int64_t nc, int64_t n, const int32_t *ptr, const int32_t *col,
                                const double *val, const double *b, const double *x,
                                double *bc, const double *done) {
   if (*done != 0.0) return;
-  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (I >= nc) return;
-  double s = 0.0;
-  for (int r = 0; r < kAgg; r++) {
-    const int64_t i = I * kAgg + r;
-    if (i >= n) break;
-    double ax = 0.0;
-    for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) ax += val[q] * x[col[q]];
-    s += b[i] - ax;
-  }
-  bc[I] = s;
+  constexpr int kGroup = kAgg * kRowLanes;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t I = t / kGroup;
+  const int lane = (int)(t % kGroup);
+  const int64_t i = I * kAgg + lane / kRowLanes;
+  const bool valid = I < nc && i < n;
+  const double ax = row_dot(ptr, col, val, x, i, valid, lane % kRowLanes);
+  double s = (valid && lane % kRowLanes == 0) ? b[i] - ax : 0.0;
+  for (int o = kRowLanes; o < kGroup; o <<= 1) s += __shfl_xor(s, o, kGroup);
+  if (I < nc && lane == 0) bc[I] = s;
 }
 // x += corr * xc[i / kAgg]
 __global__ void k_amg_prolong(int64_t n, const double *xc, double *x, const double *done) {
@@ -161,27 +161,37 @@ __global__ void k_amg_prolong(int64_t n, const double *xc, double *x, const doub
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) x[i] += kCorr * xc[i / kAgg];
 }
-// coarsest: x = L^-T L^-1 b (one workgroup)
-__global__ void __launch_bounds__(256) k_amg_dense_solve(int64_t n, const double *Lf,
+// Inverse of the coarsest matrix from its Cholesky factor L (row-major,
+// lower): thread j solves L L^T x = e_j, column j of the inverse.
+__global__ void __launch_bounds__(256) k_amg_dense_inv(int64_t n, const double *Lf,
+                                                       double *inv) {
+  const int64_t j = threadIdx.x;
+  if (j >= n) return;
+  double *x = inv + j;  // column j, stride n
+  for (int64_t i = 0; i < n; i++) {  // L y = e_j
+    double s = (i == j) ? 1.0 : 0.0;
+    for (int64_t k = j; k < i; k++) s -= Lf[i * n + k] * x[k * n];
+    x[i * n] = (i < j) ? 0.0 : s / Lf[i * n + i];
+  }
+  for (int64_t i = n - 1; i >= 0; i--) {  // L^T x = y
+    double s = x[i * n];
+    for (int64_t k = i + 1; k < n; k++) s -= Lf[k * n + i] * x[k * n];
+    x[i * n] = s / Lf[i * n + i];
+  }
+}
+// coarsest: x = A^-1 b, one row per thread (one workgroup)
+__global__ void __launch_bounds__(256) k_amg_dense_solve(int64_t n, const double *inv,
                                                          const double *b, double *x,
                                                          const double *done) {
   if (*done != 0.0) return;
-  __shared__ double y[kCoarsest];
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) y[i] = b[i];
+  __shared__ double sb[kCoarsest];
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) sb[i] = b[i];
   __syncthreads();
-  for (int64_t k = 0; k < n; k++) {
-    if (threadIdx.x == 0) y[k] /= Lf[k * n + k];
-    __syncthreads();
-    for (int64_t i = k + 1 + threadIdx.x; i < n; i += blockDim.x) y[i] -= Lf[i * n + k] * y[k];
-    __syncthreads();
+  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+    double s = 0.0;
+    for (int64_t k = 0; k < n; k++) s += inv[i * n + k] * sb[k];
+    x[i] = s;
   }
-  for (int64_t k = n - 1; k >= 0; k--) {
-    if (threadIdx.x == 0) y[k] /= Lf[k * n + k];
-    __syncthreads();
-    for (int64_t i = threadIdx.x; i < k; i += blockDim.x) y[i] -= Lf[k * n + i] * y[k];
-    __syncthreads();
-  }
-  for (int64_t i = threadIdx.x; i < n; i += blockDim.x) x[i] = y[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -195,7 +205,8 @@ struct AmgLevel {
 
 struct Amg {
   std::vector<AmgLevel> lv;
-  DBuf<double> dense;
+  DBuf<double> dense;      // Cholesky factor of the coarsest matrix
+  DBuf<double> dense_inv;  // its inverse (applied as one matrix-vector product)
   int nlev = 0;
   const int32_t *ptr0 = nullptr, *col0 = nullptr;  // level-0 pattern (the caller's CSR)
 };
@@ -265,8 +276,11 @@ ma_status amg_setup(ma_engine *e, Amg &A, const CsrView &M, int32_t gi, int max_
   const int32_t *cptr = (used == 1) ? M.ptr : Lc.ptr.p;
   const int32_t *ccol = (used == 1) ? M.col : Lc.col.p;
   MA_HIP_TRY(A.dense.ensure((size_t)(Lc.n * Lc.n)));
+  MA_HIP_TRY(A.dense_inv.ensure((size_t)(Lc.n * Lc.n)));
   hipLaunchKernelGGL(k_amg_dense_chol, dim3(1), dim3(256), 0, e->stream, Lc.n, cptr, ccol,
                      Lc.val.p, A.dense.p);
+  hipLaunchKernelGGL(k_amg_dense_inv, dim3(1), dim3(256), 0, e->stream, Lc.n, A.dense.p,
+                     A.dense_inv.p);
   A.nlev = (int)used;
   MA_HIP_TRY(hipGetLastError());
   return MA_OK;
@@ -284,18 +298,18 @@ hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const dou
     double *x = (l == 0) ? z : L.x.p;
     hipLaunchKernelGGL(k_amg_smooth0, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, L.dinv.p, b,
                        L.y.p, done);
-    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, ptr_of(l),
-                       col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done);
-    hipLaunchKernelGGL(k_amg_restrict, dim3(nb2(A.lv[l + 1].n)), dim3(kVB2), 0, e->stream,
-                       A.lv[l + 1].n, L.n, ptr_of(l), col_of(l), L.val.p, b, x,
+    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
+                       ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done);
+    hipLaunchKernelGGL(k_amg_restrict, dim3(nb2(A.lv[l + 1].n * kAgg * kRowLanes)), dim3(kVB2), 0,
+                       e->stream, A.lv[l + 1].n, L.n, ptr_of(l), col_of(l), L.val.p, b, x,
                        A.lv[l + 1].b.p, done);
   }
   {
     AmgLevel &Lc = A.lv[nl - 1];
     const double *b = (nl == 1) ? r : Lc.b.p;
     double *x = (nl == 1) ? z : Lc.x.p;
-    hipLaunchKernelGGL(k_amg_dense_solve, dim3(1), dim3(256), 0, e->stream, Lc.n, A.dense.p, b,
-                       x, done);
+    hipLaunchKernelGGL(k_amg_dense_solve, dim3(1), dim3(256), 0, e->stream, Lc.n,
+                       A.dense_inv.p, b, x, done);
   }
   // upward
   for (int l = nl - 2; l >= 0; l--) {
@@ -304,10 +318,10 @@ hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const dou
     double *x = (l == 0) ? z : L.x.p;
     hipLaunchKernelGGL(k_amg_prolong, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n,
                        A.lv[l + 1].x.p, x, done);
-    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, ptr_of(l),
-                       col_of(l), L.val.p, L.dinv.p, b, x, L.y.p, done);
-    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, ptr_of(l),
-                       col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done);
+    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
+                       ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, x, L.y.p, done);
+    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
+                       ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done);
   }
   return hipGetLastError();
 }

@@ -33,6 +33,9 @@ constexpr int kMaxFast = 16;
 constexpr int kCellBlock = 128;
 constexpr int kLocalRings = 3;  // site-centric search radius, in bins
 constexpr int kWalkBudget = 192;  // pyramid nodes a cell may visit before the bin path
+constexpr int kNodeDiracs = 8;    // pyramid nodes with at most this many Diracs are taken whole
+constexpr double kBoxBins = 8.0;  // FP32 starting box half-width, in bins
+constexpr int kBoxLabel = -3;     // edge label of that box's cut
 
 // Tolerances per arithmetic type (all relative to the local scale):
 //   snap  side of a triangle edge treated as zero (vertex on the edge line);
@@ -48,7 +51,8 @@ struct Tol<double> {
 };
 template <>
 struct Tol<float> {
-  static constexpr float snap = 1e-6f, area = 4e-6f, redo = 1e-6f;
+  // a few float ulps (2^-24 = 6e-8) of the local scale
+  static constexpr float snap = 2.5e-7f, area = 5e-7f, redo = 1e-6f;
   static constexpr double pad = 1e-5;
 };
 
@@ -56,11 +60,12 @@ struct CellArgs {
   Grid g;
   int64_t n;
   const double *pos, *w;
-  const int32_t *perm, *rect, *cand_cnt, *cand;
-  int cap;
+  const int32_t *perm, *rect;
+  const int32_t *cand_start, *cand;  // per-bin candidate owners (CSR)
   const int32_t *tbin_start, *tbin_items, *bin_start;
   double dom[4];     // domain bounding box
   const TriData *tri;
+  const float4 *tbox;  // FP32 outward-rounded triangle bounding boxes
   double *px, *py;   // FP64 cell vertices (local to the Dirac), [v * n + k]
   float *pxf, *pyf;  // FP32 cell vertices (mixed mode)
   int32_t *plab;     // neighbour across edge v -> v+1 (-1: domain edge)
@@ -105,13 +110,43 @@ struct CellLDS {
     for (int v = 0; v < n; v++) r = fmax(r, sx[v][t] * sx[v][t] + sy[v][t] * sy[v][t]);
     return (double)r * (1.0 + Tol<R>::pad);
   }
-  __device__ void rect(R X0, R Y0, R X1, R Y1) {
-    sx[0][t] = X0; sy[0][t] = Y0; sl[0][t] = -1;
-    sx[1][t] = X1; sy[1][t] = Y0; sl[1][t] = -1;
-    sx[2][t] = X1; sy[2][t] = Y1; sl[2][t] = -1;
-    sx[3][t] = X0; sy[3][t] = Y1; sl[3][t] = -1;
+  // Box [X0,X1] x [Y0,Y1] (local coordinates) with the labels of its bottom,
+  // right, top and left edges.  In FP32 the box is also cut to kBoxBins bins
+  // around the Dirac (edges labelled kBoxLabel): every vertex then stays
+  // within a few cells of the Dirac, where FP32 local coordinates are
+  // accurate (a vertex on a box edge at distance ~1 would carry an absolute
+  // error of ~1e-7, i.e. ~1e-4 of a cell).  Returns false if nothing is left.
+  __device__ bool init(double X0, double Y0, double X1, double Y1, double cut) {
+    int lb = -1, lr = -1, lt = -1, ll = -1;
+    if (X0 < -cut) { X0 = -cut; ll = kBoxLabel; }
+    if (X1 > cut) { X1 = cut; lr = kBoxLabel; }
+    if (Y0 < -cut) { Y0 = -cut; lb = kBoxLabel; }
+    if (Y1 > cut) { Y1 = cut; lt = kBoxLabel; }
+    if (!(X0 < X1 && Y0 < Y1)) return false;
+    cut = (lb == kBoxLabel) || (lr == kBoxLabel) || (lt == kBoxLabel) || (ll == kBoxLabel);
+    sx[0][t] = (R)X0; sy[0][t] = (R)Y0; sl[0][t] = lb;
+    sx[1][t] = (R)X1; sy[1][t] = (R)Y0; sl[1][t] = lr;
+    sx[2][t] = (R)X1; sy[2][t] = (R)Y1; sl[2][t] = lt;
+    sx[3][t] = (R)X0; sy[3][t] = (R)Y1; sl[3][t] = ll;
+    return true;
   }
+  // true if the finished cell may extend beyond the cut of init(): it still
+  // touches the cut, or it came out empty although the box was cut (a cell
+  // displaced from its Dirac by more than the box is not seen from inside it)
+  __device__ bool touches_cut(int n) const {
+    if (n == 0) return cut;
+    for (int v = 0; v < n; v++)
+      if (sl[v][t] == kBoxLabel) return true;
+    return false;
+  }
+  bool cut = false;
 };
+
+// Half-width of the FP32 starting box, in bins (FP64: unbounded).
+template <class R>
+__device__ inline double box_cut(const Grid &g) {
+  return sizeof(R) == sizeof(float) ? kBoxBins * g.h : kInf;
+}
 
 // Half-plane of competitor j seen from Dirac k, in coordinates local to y_k:
 // X stays in the cell of k iff kk - 2 X.Y > 0 (strict: ties are outside),
@@ -274,7 +309,7 @@ __device__ inline void write_cell(const CellArgs &a, int64_t k, double yx, doubl
   a.bbox[4 * k + 2] = bx1;
   a.bbox[4 * k + 3] = by1;
   int cnt = 0;
-  for_each_triangle(a.g, a.tbin_start, a.tbin_items, a.tri, bx0, by0, bx1, by1, 0, 0,
+  for_each_triangle(a.g, a.tbin_start, a.tbin_items, a.tbox, bx0, by0, bx1, by1, 0, 0,
                     a.g.G - 1, a.g.G - 1, [&](int32_t) { cnt++; });
   a.tcount[k] = cnt;
 }
@@ -331,9 +366,10 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
     return;
   }
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1], wk = a.w[k];
-  L.rect((R)(a.dom[0] - yx), (R)(a.dom[1] - yy), (R)(a.dom[2] - yx), (R)(a.dom[3] - yy));
   int n = 4, nredo = 0;
-  bool fail = false, certified = false;
+  bool fail = !L.init(a.dom[0] - yx, a.dom[1] - yy, a.dom[2] - yx, a.dom[3] - yy,
+                      box_cut<R>(g));
+  bool certified = false;
   double rho2 = L.rho2(4);
   const double W = pyr[off.o[g.L]];
   const int sbx = bin_coord(yx, g.x0, g.h, g.G), sby = bin_coord(yy, g.y0, g.h, g.G);
@@ -386,8 +422,12 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
           const double dx = fmax(fmax(bx0 - yx, yx - bx1), 0.0);
           const double dy = fmax(fmax(by0 - yy, yy - by1), 0.0);
           if (!far_enough(sqrt(dx * dx + dy * dy), sqrt(rho2), Wn, wk)) {
-            if (l == 0) {
-              for (int32_t q = a.bin_start[m]; q < a.bin_start[m + 1] && n > 0 && !fail; q++)
+            // the Diracs of a node are one contiguous Morton range: a node
+            // holding few of them (sparse regions) is taken whole
+            const int32_t q0 = a.bin_start[(int64_t)m << (2 * l)];
+            const int32_t q1 = a.bin_start[((int64_t)m + 1) << (2 * l)];
+            if (l == 0 || q1 - q0 <= kNodeDiracs) {
+              for (int32_t q = q0; q < q1 && n > 0 && !fail; q++)
                 consider(a, k, q, yx, yy, wk, L, n, rho2, fail, nredo);
             } else {
               descend = true;
@@ -408,6 +448,8 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
       m++;
     }
   }
+  // a cell still touching the FP32 starting box is larger than that box
+  if (!fail && !over_budget && L.touches_cut(n)) fail = true;
   wave_count(&a.flags[2], walk);  // cells that needed the pyramid walk
   wave_count(&a.counters[6], over_budget && !fail);  // cells left to the bin path
   if (Tol<R>::redo > R(0) && nredo > 0) atomicAdd(&a.counters[5], (unsigned long long)nredo);
@@ -449,10 +491,9 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_bins(CellArgs a, const int3
     return;
   }
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1], wk = a.w[k];
-  L.rect((R)(g.x0 + rx0 * g.h - yx), (R)(g.y0 + ry0 * g.h - yy),
-         (R)(g.x0 + (rx1 + 1) * g.h - yx), (R)(g.y0 + (ry1 + 1) * g.h - yy));
   int n = 4, nredo = 0;
-  bool fail = false;
+  bool fail = !L.init(g.x0 + rx0 * g.h - yx, g.y0 + ry0 * g.h - yy,
+                      g.x0 + (rx1 + 1) * g.h - yx, g.y0 + (ry1 + 1) * g.h - yy, box_cut<R>(g));
   double rho2 = L.rho2(4);
   const int sbx = min(max(bin_coord(yx, g.x0, g.h, g.G), rx0), rx1);
   const int sby = min(max(bin_coord(yy, g.y0, g.h, g.G), ry0), ry1);
@@ -463,14 +504,13 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_bins(CellArgs a, const int3
       for (int bx = max(sbx - r, rx0); bx <= min(sbx + r, rx1) && n > 0 && !fail; bx++) {
         if (!edge_row && bx != sbx - r && bx != sbx + r) continue;
         const int64_t b = morton2((uint32_t)bx, (uint32_t)by);
-        const int c = a.cand_cnt[b];
-        const int32_t *cl = a.cand + b * (int64_t)a.cap;
-        for (int i = 0; i < c && n > 0 && !fail; i++)
-          consider(a, k, cl[i], yx, yy, wk, L, n, rho2, fail, nredo);
+        for (int32_t i = a.cand_start[b]; i < a.cand_start[b + 1] && n > 0 && !fail; i++)
+          consider(a, k, a.cand[i], yx, yy, wk, L, n, rho2, fail, nredo);
       }
     }
   }
   if (Tol<R>::redo > R(0) && nredo > 0) atomicAdd(&a.counters[5], (unsigned long long)nredo);
+  if (!fail && L.touches_cut(n)) fail = true;
   if (fail) {
     a.pn[k] = -1;
     a.tcount[k] = 0;
@@ -485,7 +525,7 @@ __global__ void k_pairs(CellArgs a, const int32_t *poff, int32_t *pair_cell, int
   if (k >= a.n) return;
   if (a.tcount[k] == 0) return;
   int32_t o = poff[k];
-  for_each_triangle(a.g, a.tbin_start, a.tbin_items, a.tri, a.bbox[4 * k], a.bbox[4 * k + 1],
+  for_each_triangle(a.g, a.tbin_start, a.tbin_items, a.tbox, a.bbox[4 * k], a.bbox[4 * k + 1],
                     a.bbox[4 * k + 2], a.bbox[4 * k + 3], 0, 0, a.g.G - 1, a.g.G - 1,
                     [&](int32_t ti) {
                       pair_cell[o] = (int32_t)k;
@@ -953,12 +993,12 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   a.w = B.w_s.p;
   a.perm = e->perm.p;
   a.rect = B.rect.p;
-  a.cand_cnt = B.cand_cnt.p;
+  a.cand_start = B.cand_cnt.p;
   a.cand = B.cand.p;
-  a.cap = e->cap_cand;
   a.tbin_start = e->tbin_start.p;
   a.tbin_items = e->tbin_items.p;
   a.tri = e->tri.p;
+  a.tbox = e->tbox.p;
   a.px = B.px.p;
   a.py = B.py.p;
   a.pxf = B.pxf.p;
@@ -1003,8 +1043,8 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
       MA_HIP_TRY(hipEventRecord(e->ev[8], e->stream));
       ma_status s = run_bin_candidates(e, B.pn.p);
       if (s != MA_OK) return s;
-      a.cand = B.cand.p;  // the lists may have been regrown
-      a.cap = e->cap_cand;
+      a.cand_start = B.cand_cnt.p;  // (re)allocated by the candidate search
+      a.cand = B.cand.p;
       if (ct0[6]) {
         MA_HIP_TRY(hipMemcpyAsync(B.glist.p + n, &zero, sizeof(int32_t), hipMemcpyHostToDevice,
                                   e->stream));

@@ -64,14 +64,17 @@ struct TriData {
   double a, b, c;
 };
 
-// Capacity of the fixed-size per-thread polygon buffers.
-constexpr int kMaxPoly = 32;
+// Capacity of the general kernel's per-thread polygons, and of the staging
+// of one Hessian row (neighbours + diagonal).
+constexpr int kMaxPoly = 64;
+constexpr int kMaxRow = 65;
 
 struct EvalBuffers {
   DBuf<double> w_s;          // weights in Morton order
   DBuf<double> pyr;          // max-weight pyramid (all levels)
-  DBuf<int32_t> cand_cnt;    // per bin
-  DBuf<int32_t> cand;        // per bin x cap_cand
+  DBuf<int32_t> cand_cnt;    // CSR offsets of the per-bin candidate lists (nbins + 1)
+  DBuf<int32_t> cand;        // candidate owners, bin after bin
+  DBuf<double> ubin;         // per bin: min over Diracs of the max power over the bin
   DBuf<int32_t> owner;       // per bin, best competitor of the last evaluation
   DBuf<int32_t> rect;        // per Dirac: bin rectangle (x0, y0, x1, y1)
   DBuf<double> mass_s, intg_s, fval_part;
@@ -112,6 +115,7 @@ struct ma_engine {
   double dom[4] = {0, 0, 0, 0};  // domain bounding box
   ma::DBuf<ma::TriData> tri;
   ma::DBuf<int32_t> triv;  // ccw vertex ids (nt x 3)
+  ma::DBuf<float4> tbox;   // FP32 bounding boxes, rounded outwards
   ma::DBuf<double> vxy, vval;
   bool have_mesh = false;
 
@@ -131,7 +135,7 @@ struct ma_engine {
   int force_general = 0;  // MA_OPT_FORCE_GENERAL
   int use_graphs = 1;     // MA_OPT_GRAPHS: replay CG batches from a HIP graph
   bool ran_candidates = false;  // the last evaluation built per-bin candidate lists
-  int cap_row = ma::kMaxPoly + 1;
+  int cap_row = ma::kMaxRow;
 
   ma::EvalBuffers eb;
   ma::SolverBuffers sb;

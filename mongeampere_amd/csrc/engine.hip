@@ -8,7 +8,7 @@
 #include <cstring>
 #include <limits>
 
-#include "engine.h"
+#include "geom_common.h"
 
 namespace ma {
 
@@ -22,7 +22,7 @@ constexpr int kVGrid = 512;   // fixed grid => fixed number of partial sums
 // (the plane through the three vertex values; include/MA/functions.hpp:55-80).
 // ---------------------------------------------------------------------------
 __global__ void k_tri_prep(const double *vxy, const double *vval, const int32_t *tri, int64_t nt,
-                           TriData *out, int32_t *triv, int32_t *bad) {
+                           TriData *out, int32_t *triv, float4 *tbox, int32_t *bad) {
   int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (f >= nt) return;
   int32_t id[3] = {tri[3 * f], tri[3 * f + 1], tri[3 * f + 2]};
@@ -54,6 +54,10 @@ __global__ void k_tri_prep(const double *vxy, const double *vval, const int32_t 
   T.c = v[0] - T.a * x[0] - T.b * y[0];
   out[f] = T;
   for (int k = 0; k < 3; k++) triv[3 * f + k] = id[k];
+  tbox[f] = make_float4(__double2float_rd(fmin(x[0], fmin(x[1], x[2]))),
+                        __double2float_rd(fmin(y[0], fmin(y[1], y[2]))),
+                        __double2float_ru(fmax(x[0], fmax(x[1], x[2]))),
+                        __double2float_ru(fmax(y[0], fmax(y[1], y[2]))));
 }
 
 // ---------------------------------------------------------------------------
@@ -91,11 +95,15 @@ __global__ void __launch_bounds__(kVB) k_spmv_dot(int64_t n, const int32_t *ptr,
   __shared__ double s[kVB];
   if (scal[5] != 0.0) return;
   double acc = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
-    double v = 0.0;
-    for (int32_t q = ptr[i]; q < ptr[i + 1]; q++) v += val[q] * x[col[q]];
-    y[i] = v;
-    acc += x[i] * v;
+  const int lane = (int)(threadIdx.x % kRowLanes);
+  const int64_t stride = (int64_t)kVGrid * kVB / kRowLanes;
+  // the kRowLanes lanes of a group share i, so they iterate together
+  for (int64_t i = (blockIdx.x * (int64_t)kVB + threadIdx.x) / kRowLanes; i < n; i += stride) {
+    const double v = row_dot(ptr, col, val, x, i, true, lane);
+    if (lane == 0) {
+      y[i] = v;
+      acc += x[i] * v;
+    }
   }
   acc = block_sum(acc, s);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -519,6 +527,7 @@ ma_status ma_set_mesh(ma_engine *e, int64_t nv, const double *vxy, const double 
   MA_HIP_TRY(e->vval.ensure((size_t)nv));
   MA_HIP_TRY(e->tri.ensure((size_t)nt));
   MA_HIP_TRY(e->triv.ensure((size_t)(3 * nt)));
+  MA_HIP_TRY(e->tbox.ensure((size_t)nt));
   DBuf<int32_t> dtri, bad;
   MA_HIP_TRY(dtri.ensure((size_t)(3 * nt)));
   MA_HIP_TRY(bad.ensure(1));
@@ -527,7 +536,7 @@ ma_status ma_set_mesh(ma_engine *e, int64_t nv, const double *vxy, const double 
   MA_HIP_TRY(hipMemcpyAsync(dtri.p, tri, sizeof(int32_t) * 3 * nt, hipMemcpyHostToDevice, e->stream));
   MA_HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int32_t), e->stream));
   hipLaunchKernelGGL(k_tri_prep, dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, e->stream,
-                     e->vxy.p, e->vval.p, dtri.p, nt, e->tri.p, e->triv.p, bad.p);
+                     e->vxy.p, e->vval.p, dtri.p, nt, e->tri.p, e->triv.p, e->tbox.p, bad.p);
   int32_t hb = 0;
   MA_HIP_TRY(hipMemcpyAsync(&hb, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
   MA_HIP_TRY(hipStreamSynchronize(e->stream));
@@ -899,6 +908,25 @@ ma_status ma_set_option(ma_engine *e, int option, int64_t value) {
 ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info) {
   if (!e || !info) return MA_BAD_INPUT;
   *info = e->info;
+  return MA_OK;
+}
+
+ma_status ma_cell_paths(ma_engine *e, int64_t count, const int64_t *ids, int32_t *out) {
+  if (!e || count < 0 || (count > 0 && (!ids || !out))) return MA_BAD_INPUT;
+  if (!e->have_eval) {
+    e->err = "no evaluation yet";
+    return MA_NO_DATA;
+  }
+  std::vector<int32_t> pn(e->n), iperm(e->n);
+  MA_HIP_TRY(hipMemcpyAsync(pn.data(), e->eb.pn.p, sizeof(int32_t) * e->n, hipMemcpyDeviceToHost,
+                            e->stream));
+  MA_HIP_TRY(hipMemcpyAsync(iperm.data(), e->iperm.p, sizeof(int32_t) * e->n,
+                            hipMemcpyDeviceToHost, e->stream));
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  for (int64_t i = 0; i < count; i++) {
+    if (ids[i] < 0 || ids[i] >= e->n) return MA_BAD_INPUT;
+    out[i] = std::max(pn[iperm[ids[i]]], -1);
+  }
   return MA_OK;
 }
 

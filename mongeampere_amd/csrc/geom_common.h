@@ -38,6 +38,21 @@ __device__ inline int bin_coord(double v, double o, double h, int G) {
 }
 static inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 
+// Sparse row times vector by a group of kRowLanes consecutive lanes: lane l
+// takes entries l, l + kRowLanes, ... of row i, so a wave reads consecutive
+// entries of consecutive rows (coalesced); every lane of the group returns
+// the sum.  All lanes of the group must call it (valid = false: no row).
+constexpr int kRowLanes = 8;
+__device__ inline double row_dot(const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                                 const double *__restrict__ val, const double *__restrict__ x,
+                                 int64_t i, bool valid, int lane) {
+  double s = 0.0;
+  if (valid)
+    for (int32_t q = ptr[i] + lane; q < ptr[i + 1]; q += kRowLanes) s += val[q] * x[col[q]];
+  for (int o = 1; o < kRowLanes; o <<= 1) s += __shfl_xor(s, o, kRowLanes);
+  return s;
+}
+
 __device__ inline void tri_bbox(const TriData &t, double &x0, double &y0, double &x1,
                                 double &y1) {
   x0 = fmin(t.x[0], fmin(t.x[1], t.x[2]));
@@ -81,10 +96,12 @@ __device__ inline void ac6_points(R ax, R ay, R bx, R by, R cx, R cy, AC6T<R> &q
 // the bin rectangle [rx0,rx1]x[ry0,ry1].  A triangle listed in several bins
 // is visited in the bin holding the lower-left corner of the overlap of the
 // two boxes.
+// tbox: per triangle its bounding box (x0, y0, x1, y1) in FP32, rounded
+// outwards (16 B per test instead of the 72 B triangle record).
 template <class F>
 __device__ inline void for_each_triangle(const Grid &g, const int32_t *__restrict__ tbin_start,
                                          const int32_t *__restrict__ tbin_items,
-                                         const TriData *__restrict__ tri, double px0, double py0,
+                                         const float4 *__restrict__ tbox, double px0, double py0,
                                          double px1, double py1, int rx0, int ry0, int rx1,
                                          int ry1, F f) {
   const int qx0 = max(bin_coord(px0, g.x0, g.h, g.G), rx0);
@@ -96,9 +113,8 @@ __device__ inline void for_each_triangle(const Grid &g, const int32_t *__restric
       const int64_t b = morton2((uint32_t)bx, (uint32_t)by);
       for (int32_t q = tbin_start[b]; q < tbin_start[b + 1]; q++) {
         const int32_t ti = tbin_items[q];
-        const TriData &t = tri[ti];
-        double tx0, ty0, tx1, ty1;
-        tri_bbox(t, tx0, ty0, tx1, ty1);
+        const float4 tb = tbox[ti];
+        const double tx0 = tb.x, ty0 = tb.y, tx1 = tb.z, ty1 = tb.w;
         if (tx1 < px0 || tx0 > px1 || ty1 < py0 || ty0 > py1) continue;
         const int ox = max(bin_coord(fmax(tx0, px0), g.x0, g.h, g.G), qx0);
         const int oy = max(bin_coord(fmax(ty0, py0), g.y0, g.h, g.G), qy0);

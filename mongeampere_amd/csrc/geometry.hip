@@ -235,12 +235,14 @@ hipError_t build_dirac_grid(ma_engine *e, const double *d_X, int layout) {
 // Triangle bins: every triangle is listed in each bin its bounding box meets.
 // ---------------------------------------------------------------------------
 template <bool FILL>
-__global__ void k_tbin(const TriData *tri, int64_t nt, Grid g, int32_t *cnt_or_cursor,
+// A triangle is listed in every bin its FP32 (outward-rounded) bounding box
+// meets: the same box for_each_triangle uses to visit it once.
+__global__ void k_tbin(const float4 *tbox, int64_t nt, Grid g, int32_t *cnt_or_cursor,
                        int32_t *items) {
   int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (f >= nt) return;
-  double x0, y0, x1, y1;
-  tri_bbox(tri[f], x0, y0, x1, y1);
+  const float4 tb = tbox[f];
+  const double x0 = tb.x, y0 = tb.y, x1 = tb.z, y1 = tb.w;
   int bx0 = bin_coord(x0, g.x0, g.h, g.G), bx1 = bin_coord(x1, g.x0, g.h, g.G);
   int by0 = bin_coord(y0, g.y0, g.h, g.G), by1 = bin_coord(y1, g.y0, g.h, g.G);
   for (int by = by0; by <= by1; by++)
@@ -261,7 +263,7 @@ hipError_t build_triangle_bins(ma_engine *e) {
   MA_HIP_RET(cnt.ensure((size_t)g.nbins + 1));
   MA_HIP_RET(e->tbin_start.ensure((size_t)g.nbins + 1));
   MA_HIP_RET(hipMemsetAsync(cnt.p, 0, sizeof(int32_t) * (g.nbins + 1), e->stream));
-  hipLaunchKernelGGL(k_tbin<false>, dim3(nblocks(e->nt)), dim3(kBlock), 0, e->stream, e->tri.p,
+  hipLaunchKernelGGL(k_tbin<false>, dim3(nblocks(e->nt)), dim3(kBlock), 0, e->stream, e->tbox.p,
                      e->nt, g, cnt.p, nullptr);
   MA_HIP_RET(exclusive_scan(e, cnt.p, e->tbin_start.p, g.nbins + 1));
   int32_t total = 0;
@@ -271,7 +273,7 @@ hipError_t build_triangle_bins(ma_engine *e) {
   MA_HIP_RET(e->tbin_items.ensure((size_t)total + 1));
   MA_HIP_RET(hipMemcpyAsync(cnt.p, e->tbin_start.p, sizeof(int32_t) * g.nbins,
                             hipMemcpyDeviceToDevice, e->stream));
-  hipLaunchKernelGGL(k_tbin<true>, dim3(nblocks(e->nt)), dim3(kBlock), 0, e->stream, e->tri.p,
+  hipLaunchKernelGGL(k_tbin<true>, dim3(nblocks(e->nt)), dim3(kBlock), 0, e->stream, e->tbox.p,
                      e->nt, g, cnt.p, e->tbin_items.p);
   hipLaunchKernelGGL(k_sort_bins, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream,
                      e->tbin_start.p, g.nbins, e->tbin_items.p);
@@ -338,118 +340,132 @@ __device__ inline bool above(double lo, double U) {
   return lo > U + 1e-12 * (fabs(lo) + fabs(U));
 }
 
-__global__ void __launch_bounds__(kBlock) k_candidates(
-    Grid g, const int32_t *__restrict__ tbin_start, const int32_t *__restrict__ bin_start,
-    const double *__restrict__ pos, const double *__restrict__ w, const double *__restrict__ pyr,
-    PyrOff off, int32_t *cand_cnt, int32_t *cand, int cap, int32_t *owner, int32_t *flags,
-    unsigned long long *counters) {
-  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (b >= g.nbins) return;
-  if (tbin_start[b + 1] == tbin_start[b]) {
-    cand_cnt[b] = 0;
-    return;
-  }
-  const uint32_t bx = compact1by1((uint32_t)b), by = compact1by1((uint32_t)(b >> 1));
-  BinGeom B;
-  B.x0 = g.x0 + bx * g.h;
-  B.y0 = g.y0 + by * g.h;
-  B.x1 = B.x0 + g.h;
-  B.y1 = B.y0 + g.h;
-  int best = owner[b];
-  double U = kInf, bx_ = 0, by_ = 0, bw_ = 0;
-  if (best >= 0) {
-    bx_ = pos[2 * best];
-    by_ = pos[2 * best + 1];
-    bw_ = w[best];
-    U = pi_max_corners(B, bx_, by_, bw_);
-  }
-  int32_t *out = cand + b * (int64_t)cap;
-  int cnt = 0;
-  bool lost = false;
+// Depth-first walk of the max-weight pyramid for bin B, nearest nodes
+// first, pruning nodes whose lower bound dist^2(node, B) - max w exceeds U
+// (U may decrease while walking); visit(q) is called for every Dirac of the
+// surviving leaves.  Returns false if the fixed-size stack overflowed.
+template <class Visit>
+__device__ inline bool pyr_walk(const Grid &g, const int32_t *bin_start, const double *pyr,
+                                const PyrOff &off, const BinGeom &B, const double &U,
+                                Visit &&visit) {
   uint32_t stack[64];
   int sp = 0;
+  bool ok = true;
   stack[sp++] = ((uint32_t)g.L << 27);
   while (sp > 0) {
     const uint32_t ent = stack[--sp];
     const int l = (int)(ent >> 27);
     const uint32_t m = ent & ((1u << 27) - 1u);
     if (l == 0) {
-      for (int32_t q = bin_start[m]; q < bin_start[m + 1]; q++) {
-        const double jx = pos[2 * q], jy = pos[2 * q + 1], wj = w[q];
-        const double hij = pi_max_corners(B, jx, jy, wj);
-        if (hij < U) {
-          U = hij;
-          best = q;
-          bx_ = jx;
-          by_ = jy;
-          bw_ = wj;
-        }
-        if (above(pi_min_rect(B, jx, jy, wj), U)) continue;
-        if (q != best && !may_beat(B, jx, jy, wj, bx_, by_, bw_)) continue;
-        if (cnt == cap) {
-          // compact against the current competitor before giving up
-          int c2 = 0;
-          for (int i = 0; i < cnt; i++) {
-            int32_t r = out[i];
-            double rx = pos[2 * r], ry = pos[2 * r + 1], rw = w[r];
-            if (r == best || (!above(pi_min_rect(B, rx, ry, rw), U) &&
-                              may_beat(B, rx, ry, rw, bx_, by_, bw_)))
-              out[c2++] = r;
-          }
-          cnt = c2;
-          if (cnt == cap) {
-            lost = true;
-            continue;
-          }
-        }
-        out[cnt++] = q;
+      for (int32_t q = bin_start[m]; q < bin_start[m + 1]; q++) visit(q);
+      continue;
+    }
+    const int cl = l - 1;
+    const uint32_t c0 = m << 2;
+    const double side = g.h * (double)(1u << cl);
+    double lo4[4];
+    uint32_t id4[4];
+    int k4 = 0;
+    for (int c = 0; c < 4; c++) {
+      const uint32_t ch = c0 + (uint32_t)c;
+      const double W = pyr[off.o[cl] + ch];
+      if (W == -kInf) continue;
+      const double nx0 = g.x0 + compact1by1(ch) * side, ny0 = g.y0 + compact1by1(ch >> 1) * side;
+      const double dx = fmax(fmax(nx0 - B.x1, B.x0 - (nx0 + side)), 0.0);
+      const double dy = fmax(fmax(ny0 - B.y1, B.y0 - (ny0 + side)), 0.0);
+      const double lo = dx * dx + dy * dy - W;
+      if (above(lo, U)) continue;
+      int p = k4++;
+      while (p > 0 && lo4[p - 1] < lo) {  // keep descending order: nearest popped first
+        lo4[p] = lo4[p - 1];
+        id4[p] = id4[p - 1];
+        p--;
       }
-    } else {
-      const int cl = l - 1;
-      const uint32_t c0 = m << 2;
-      const double side = g.h * (double)(1u << cl);
-      double lo4[4];
-      uint32_t id4[4];
-      int k4 = 0;
-      for (int c = 0; c < 4; c++) {
-        const uint32_t ch = c0 + (uint32_t)c;
-        const double W = pyr[off.o[cl] + ch];
-        if (W == -kInf) continue;
-        const double nx0 = g.x0 + compact1by1(ch) * side, ny0 = g.y0 + compact1by1(ch >> 1) * side;
-        const double dx = fmax(fmax(nx0 - B.x1, B.x0 - (nx0 + side)), 0.0);
-        const double dy = fmax(fmax(ny0 - B.y1, B.y0 - (ny0 + side)), 0.0);
-        const double lo = dx * dx + dy * dy - W;
-        if (above(lo, U)) continue;
-        int p = k4++;
-        while (p > 0 && lo4[p - 1] < lo) {  // keep descending order
-          lo4[p] = lo4[p - 1];
-          id4[p] = id4[p - 1];
-          p--;
-        }
-        lo4[p] = lo;
-        id4[p] = ch;
-      }
-      for (int c = 0; c < k4; c++) {
-        if (sp < 64)
-          stack[sp++] = ((uint32_t)cl << 27) | id4[c];
-        else
-          lost = true;
-      }
+      lo4[p] = lo;
+      id4[p] = ch;
+    }
+    for (int c = 0; c < k4; c++) {
+      if (sp < 64)
+        stack[sp++] = ((uint32_t)cl << 27) | id4[c];
+      else
+        ok = false;
     }
   }
-  // final filter against the final competitor
-  int n2 = 0;
-  for (int i = 0; i < cnt; i++) {
-    int32_t r = out[i];
-    double rx = pos[2 * r], ry = pos[2 * r + 1], rw = w[r];
-    if (r == best ||
-        (!above(pi_min_rect(B, rx, ry, rw), U) && may_beat(B, rx, ry, rw, bx_, by_, bw_)))
-      out[n2++] = r;
+  return ok;
+}
+
+__device__ inline BinGeom bin_geom(const Grid &g, int64_t b) {
+  const uint32_t bx = compact1by1((uint32_t)b), by = compact1by1((uint32_t)(b >> 1));
+  BinGeom B;
+  B.x0 = g.x0 + bx * g.h;
+  B.y0 = g.y0 + by * g.h;
+  B.x1 = B.x0 + g.h;
+  B.y1 = B.y0 + g.h;
+  return B;
+}
+
+// Candidate owners of a bin, in three passes.  A Dirac j can own a point of
+// bin b only if min_{x in b} pi_j(x) <= U_b := min_k max_{x in b} pi_k(x),
+// and only if pi_j - pi_k (affine) is <= 0 at some corner of b for the
+// minimiser k ("owner").  Pass 1 finds U_b and the owner (warm-started from
+// the previous evaluation's owner); passes 2 and 3 count and then write the
+// Diracs passing both tests, into exactly sized CSR lists.
+__global__ void __launch_bounds__(kBlock) k_cand_owner(
+    Grid g, const int32_t *__restrict__ tbin_start, const int32_t *__restrict__ bin_start,
+    const double *__restrict__ pos, const double *__restrict__ w, const double *__restrict__ pyr,
+    PyrOff off, int32_t *owner, double *ubin, int32_t *flags) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= g.nbins) return;
+  if (tbin_start[b + 1] == tbin_start[b]) {
+    ubin[b] = -kInf;  // no triangle: no candidates needed
+    return;
   }
-  cand_cnt[b] = n2;
+  const BinGeom B = bin_geom(g, b);
+  int best = owner[b];
+  double U = kInf;
+  if (best >= 0) U = pi_max_corners(B, pos[2 * best], pos[2 * best + 1], w[best]);
+  const bool ok = pyr_walk(g, bin_start, pyr, off, B, U, [&](int32_t q) {
+    const double h = pi_max_corners(B, pos[2 * q], pos[2 * q + 1], w[q]);
+    if (h < U) {
+      U = h;
+      best = q;
+    }
+  });
   owner[b] = best;
-  if (lost) atomicOr(&flags[1], 1);
-  atomicAdd(&counters[1], (unsigned long long)n2);
+  ubin[b] = U;
+  if (!ok) atomicOr(&flags[1], 1);
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(kBlock) k_cand_list(
+    Grid g, const int32_t *__restrict__ bin_start, const double *__restrict__ pos,
+    const double *__restrict__ w, const double *__restrict__ pyr, PyrOff off,
+    const int32_t *__restrict__ owner, const double *__restrict__ ubin, int32_t *cand_start,
+    int32_t *cand, int32_t *flags, unsigned long long *counters) {
+  const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (b >= g.nbins) return;
+  const double U = ubin[b];
+  if (U == -kInf) {
+    if (!FILL) cand_start[b] = 0;
+    return;
+  }
+  const BinGeom B = bin_geom(g, b);
+  const int best = owner[b];
+  const double bx = pos[2 * best], by = pos[2 * best + 1], bw = w[best];
+  int cnt = 0;
+  int32_t *out = FILL ? cand + cand_start[b] : nullptr;
+  const bool ok = pyr_walk(g, bin_start, pyr, off, B, U, [&](int32_t q) {
+    const double jx = pos[2 * q], jy = pos[2 * q + 1], wj = w[q];
+    if (q != best && (above(pi_min_rect(B, jx, jy, wj), U) || !may_beat(B, jx, jy, wj, bx, by, bw)))
+      return;
+    if (FILL) out[cnt] = q;
+    cnt++;
+  });
+  if (!FILL) {
+    cand_start[b] = cnt;
+    atomicAdd(&counters[1], (unsigned long long)cnt);
+  }
+  if (!ok) atomicOr(&flags[1], 1);
 }
 
 __global__ void k_rect_init(int32_t *r, int64_t n) {
@@ -465,15 +481,14 @@ __global__ void k_rect_init(int32_t *r, int64_t n) {
 // listing it as a candidate.
 // Only Diracs marked pn[q] < 0 (uncertified by the site-centric search, or
 // overflowing it) are needed.
-__global__ void k_rect(Grid g, const int32_t *tbin_start, const int32_t *cand_cnt,
-                       const int32_t *cand, int cap, const int32_t *pn, int32_t *rect) {
+__global__ void k_rect(Grid g, const int32_t *tbin_start, const int32_t *cand_start,
+                       const int32_t *cand, const int32_t *pn, int32_t *rect) {
   const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (b >= g.nbins) return;
   if (tbin_start[b + 1] == tbin_start[b]) return;
   const int bx = (int)compact1by1((uint32_t)b), by = (int)compact1by1((uint32_t)(b >> 1));
-  const int c = cand_cnt[b];
-  for (int i = 0; i < c; i++) {
-    int32_t q = cand[b * (int64_t)cap + i];
+  for (int32_t i = cand_start[b]; i < cand_start[b + 1]; i++) {
+    int32_t q = cand[i];
     if (pn[q] >= 0) continue;
     atomicMin(&rect[4 * q + 0], bx);
     atomicMin(&rect[4 * q + 1], by);
@@ -608,7 +623,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
     const int32_t *__restrict__ list, const int32_t *__restrict__ nlist,
     Grid g, int64_t n, const double *__restrict__ pos, const double *__restrict__ w,
     const int32_t *__restrict__ perm, const int32_t *__restrict__ rect,
-    const int32_t *__restrict__ cand_cnt, const int32_t *__restrict__ cand, int cap,
+    const int32_t *__restrict__ cand_start, const int32_t *__restrict__ cand,
     const int32_t *__restrict__ tbin_start, const int32_t *__restrict__ tbin_items,
     const TriData *__restrict__ tri, double *mass_out, double *intg_out, double *fval_part,
     double *mom_out, int32_t *row_cnt, int32_t *row_col, double *row_val, int cap_row,
@@ -633,12 +648,18 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
   bool overflow = false;
   double rho2 = 0.0;
   for (int v = 0; v < P.n; v++) rho2 = fmax(rho2, P.x[v] * P.x[v] + P.y[v] * P.y[v]);
-  for (int by = ry0; by <= ry1 && P.n > 0; by++)
-    for (int bx = rx0; bx <= rx1 && P.n > 0; bx++) {
+  // bins in Chebyshev rings around the Dirac's bin (clamped into the
+  // rectangle): near candidates first keep the intermediate polygons small
+  const int sbx = min(max(bin_coord(yx, g.x0, g.h, g.G), rx0), rx1);
+  const int sby = min(max(bin_coord(yy, g.y0, g.h, g.G), ry0), ry1);
+  const int maxr = max(max(sbx - rx0, rx1 - sbx), max(sby - ry0, ry1 - sby));
+  for (int r = 0; r <= maxr && P.n > 0; r++)
+  for (int by = max(sby - r, ry0); by <= min(sby + r, ry1) && P.n > 0; by++)
+    for (int bx = max(sbx - r, rx0); bx <= min(sbx + r, rx1) && P.n > 0; bx++) {
+      if (by != sby - r && by != sby + r && bx != sbx - r && bx != sbx + r) continue;
       const int64_t b = morton2((uint32_t)bx, (uint32_t)by);
-      const int c = cand_cnt[b];
-      for (int i = 0; i < c && P.n > 0; i++) {
-        const int32_t j = cand[b * (int64_t)cap + i];
+      for (int32_t i = cand_start[b]; i < cand_start[b + 1] && P.n > 0; i++) {
+        const int32_t j = cand[i];
         if (j == k) continue;
         bool seen = false;
         for (int v = 0; v < P.n; v++) seen |= (P.lab[v] == j);
@@ -664,7 +685,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
 
   double mass = 0.0, intg = 0.0, momx = 0.0, momy = 0.0;
   double acc[kMaxPoly];
-  uint32_t touched = 0u;
+  uint64_t touched = 0u;
   int nfrag = 0;
   const int np = P.n;
   for (int v = 0; v < np; v++) acc[v] = 0.0;
@@ -800,7 +821,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
               if (ex == 0.0 && ey == 0.0) continue;  // piece from a snapped vertex
               const double mx = 0.5 * (Q.x[i] + Q.x[i1]), my = 0.5 * (Q.y[i] + Q.y[i1]);
               acc[lab] += sqrt(ex * ex + ey * ey) * (r0 + t.a * mx + t.b * my);
-              touched |= 1u << lab;
+              touched |= 1ull << lab;
             }
           }
         }
@@ -816,7 +837,7 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
     int nr = 0;
     double diag = 0.0;
     for (int v = 0; v < np; v++) {
-      if (!((touched >> v) & 1u)) continue;
+      if (!((touched >> v) & 1ull)) continue;
       const int32_t j = P.lab[v];
       const double dx = yx - pos[2 * j], dy = yy - pos[2 * j + 1];
       const double val = acc[v] / (2.0 * sqrt(dx * dx + dy * dy));
@@ -834,6 +855,10 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
       cols[p] = j;
       vals[p] = -val;
       nr++;
+    }
+    if (nr + 1 > cap_row) {  // more neighbours than the row staging holds
+      overflow = true;
+      nr = 0;
     }
     int cnt = 0;
     if (nr > 0) {
@@ -921,25 +946,35 @@ ma_status run_bin_candidates(ma_engine *e, const int32_t *pn) {
   EvalBuffers &B = e->eb;
   PyrOff po{};
   for (int l = 0; l <= g.L; l++) po.o[l] = e->pyr_off[l];
-  for (int attempt = 0;; attempt++) {
-    MA_HIP_TRY(B.cand.ensure((size_t)g.nbins * e->cap_cand));
-    MA_HIP_TRY(hipMemsetAsync(B.flags.p + 1, 0, sizeof(int32_t), e->stream));
-    hipLaunchKernelGGL(k_candidates, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
-                       e->tbin_start.p, e->bin_start.p, e->pos.p, B.w_s.p, B.pyr.p, po,
-                       B.cand_cnt.p, B.cand.p, e->cap_cand, B.owner.p, B.flags.p, B.counters.p);
-    int32_t fl[4];
-    MA_HIP_TRY(hipMemcpyAsync(fl, B.flags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
-    MA_HIP_TRY(hipStreamSynchronize(e->stream));
-    if (!fl[1]) break;
-    if (attempt >= 3) {
-      e->err = "candidate list overflow (cap " + std::to_string(e->cap_cand) + ")";
-      return MA_CAPACITY;
-    }
-    e->cap_cand *= 2;
+  MA_HIP_TRY(B.ubin.ensure((size_t)g.nbins));
+  MA_HIP_TRY(B.cand_cnt.ensure((size_t)g.nbins + 1));
+  MA_HIP_TRY(hipMemsetAsync(B.flags.p + 1, 0, sizeof(int32_t), e->stream));
+  hipLaunchKernelGGL(k_cand_owner, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
+                     e->tbin_start.p, e->bin_start.p, e->pos.p, B.w_s.p, B.pyr.p, po, B.owner.p,
+                     B.ubin.p, B.flags.p);
+  hipLaunchKernelGGL(k_cand_list<false>, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
+                     e->bin_start.p, e->pos.p, B.w_s.p, B.pyr.p, po, B.owner.p, B.ubin.p,
+                     B.cand_cnt.p, (int32_t *)nullptr, B.flags.p, B.counters.p);
+  const int32_t zero = 0;
+  MA_HIP_TRY(hipMemcpyAsync(B.cand_cnt.p + g.nbins, &zero, sizeof(int32_t),
+                            hipMemcpyHostToDevice, e->stream));
+  MA_HIP_TRY(exclusive_scan(e, B.cand_cnt.p, B.cand_cnt.p, g.nbins + 1));
+  int32_t total = 0, fl[4];
+  MA_HIP_TRY(hipMemcpyAsync(&total, B.cand_cnt.p + g.nbins, sizeof(int32_t),
+                            hipMemcpyDeviceToHost, e->stream));
+  MA_HIP_TRY(hipMemcpyAsync(fl, B.flags.p, sizeof fl, hipMemcpyDeviceToHost, e->stream));
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  if (fl[1]) {
+    e->err = "candidate search stack overflow";
+    return MA_CAPACITY;
   }
+  MA_HIP_TRY(B.cand.ensure((size_t)total + 1));
+  hipLaunchKernelGGL(k_cand_list<true>, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
+                     e->bin_start.p, e->pos.p, B.w_s.p, B.pyr.p, po, B.owner.p, B.ubin.p,
+                     B.cand_cnt.p, B.cand.p, B.flags.p, B.counters.p);
   hipLaunchKernelGGL(k_rect_init, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, B.rect.p, n);
   hipLaunchKernelGGL(k_rect, dim3(nblocks(g.nbins)), dim3(kBlock), 0, e->stream, g,
-                     e->tbin_start.p, B.cand_cnt.p, B.cand.p, e->cap_cand, pn, B.rect.p);
+                     e->tbin_start.p, B.cand_cnt.p, B.cand.p, pn, B.rect.p);
   MA_HIP_TRY(hipGetLastError());
   return MA_OK;
 }
@@ -958,7 +993,7 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   PyrOff po{};
   for (int l = 0; l <= g.L; l++) po.o[l] = e->pyr_off[l];
   MA_HIP_TRY(B.pyr.ensure((size_t)tot));
-  MA_HIP_TRY(B.cand_cnt.ensure((size_t)g.nbins));
+  MA_HIP_TRY(B.cand_cnt.ensure((size_t)g.nbins + 1));
   MA_HIP_TRY(B.owner.ensure((size_t)g.nbins));
   MA_HIP_TRY(B.rect.ensure((size_t)(4 * n)));
   MA_HIP_TRY(B.mass_s.ensure((size_t)n));
@@ -997,7 +1032,7 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
                                                             : k_cells_general<2>;
     hipLaunchKernelGGL(kern, dim3(nblocks(ngen)), dim3(kBlock), 0, e->stream, B.glist.p,
                        B.glist.p + n, g, n, e->pos.p, B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p,
-                       B.cand.p, e->cap_cand, e->tbin_start.p, e->tbin_items.p, e->tri.p,
+                       B.cand.p, e->tbin_start.p, e->tbin_items.p, e->tri.p,
                        B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p, B.row_cnt.p,
                        B.row_col.p, B.row_val.p, e->cap_row, e->triv.p, B.dval.p, B.flags.p,
                        B.counters.p);
@@ -1054,7 +1089,8 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   e->info.ngeneral = ngen;
   e->info.nnz = e->nnz;
   if (fl[0]) {
-    e->err = "cell polygon exceeded " + std::to_string(kMaxPoly) + " vertices";
+    e->err = "cell polygon exceeded " + std::to_string(kMaxPoly) + " vertices or " +
+             std::to_string(kMaxRow - 1) + " neighbours";
     return MA_CAPACITY;
   }
   e->have_eval = true;
