@@ -95,6 +95,8 @@ typedef struct {
   int64_t nredecide;    /* mixed mode: FP32 clip tests re-decided in FP64     */
   int64_t nbincand;     /* cells rebuilt from per-bin candidate owners (walk
                            over budget)                                       */
+  int64_t nhuge;        /* cells beyond the general kernel's 60 vertices,
+                           evaluated one workgroup per cell                   */
 } ma_eval_info;
 
 ma_status ma_create(int device, ma_engine **out);
@@ -179,9 +181,10 @@ ma_status ma_get_eval_info(const ma_engine *e, ma_eval_info *info);
  * the general kernel. */
 ma_status ma_cell_paths(ma_engine *e, int64_t count, const int64_t *ids, int32_t *out);
 
-/* Engine options.  MA_OPT_FORCE_GENERAL (0/1): evaluate every cell with the
- * general kernel instead of the register/LDS fast path (for testing the
- * overflow path); MA_OPT_CAP_CAND: initial capacity of the per-bin candidate
+/* Engine options.  MA_OPT_FORCE_GENERAL (0/1/2): evaluate every cell with the
+ * general kernel instead of the register/LDS fast path (1), or with the
+ * one-workgroup-per-cell kernel for large cells (2) (for testing the overflow
+ * paths); MA_OPT_CAP_CAND: initial capacity of the per-bin candidate
  * lists (grows automatically on overflow); MA_OPT_GRAPHS (0/1, default 1):
  * replay batches of conjugate-gradient iterations from a HIP graph. */
 typedef enum { MA_OPT_FORCE_GENERAL = 1, MA_OPT_CAP_CAND = 2, MA_OPT_GRAPHS = 3 } ma_option;

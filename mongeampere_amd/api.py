@@ -81,6 +81,11 @@ class Engine:
     def __exit__(self, *a):
         self.close()
 
+    def last_error(self):
+        """Message of the last failed call on this engine."""
+        msg = self._L.ma_last_error(self._h)
+        return msg.decode() if msg else ""
+
     def _check(self, st):
         if st != 0:
             msg = self._L.ma_last_error(self._h)

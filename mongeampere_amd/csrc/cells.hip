@@ -116,14 +116,14 @@ struct CellLDS {
   // within a few cells of the Dirac, where FP32 local coordinates are
   // accurate (a vertex on a box edge at distance ~1 would carry an absolute
   // error of ~1e-7, i.e. ~1e-4 of a cell).  Returns false if nothing is left.
-  __device__ bool init(double X0, double Y0, double X1, double Y1, double cut) {
+  __device__ bool init(double X0, double Y0, double X1, double Y1, double half) {
     int lb = -1, lr = -1, lt = -1, ll = -1;
-    if (X0 < -cut) { X0 = -cut; ll = kBoxLabel; }
-    if (X1 > cut) { X1 = cut; lr = kBoxLabel; }
-    if (Y0 < -cut) { Y0 = -cut; lb = kBoxLabel; }
-    if (Y1 > cut) { Y1 = cut; lt = kBoxLabel; }
-    if (!(X0 < X1 && Y0 < Y1)) return false;
+    if (X0 < -half) { X0 = -half; ll = kBoxLabel; }
+    if (X1 > half) { X1 = half; lr = kBoxLabel; }
+    if (Y0 < -half) { Y0 = -half; lb = kBoxLabel; }
+    if (Y1 > half) { Y1 = half; lt = kBoxLabel; }
     cut = (lb == kBoxLabel) || (lr == kBoxLabel) || (lt == kBoxLabel) || (ll == kBoxLabel);
+    if (!(X0 < X1 && Y0 < Y1)) return false;
     sx[0][t] = (R)X0; sy[0][t] = (R)Y0; sl[0][t] = lb;
     sx[1][t] = (R)X1; sy[1][t] = (R)Y0; sl[1][t] = lr;
     sx[2][t] = (R)X1; sy[2][t] = (R)Y1; sl[2][t] = lt;
@@ -488,6 +488,13 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_bins(CellArgs a, const int3
   if (rx0 > rx1) {
     a.pn[k] = 0;
     a.tcount[k] = 0;
+    return;
+  }
+  if ((int64_t)(rx1 - rx0 + 1) * (ry1 - ry0 + 1) > kBigRect) {
+    // a long sequential candidate loop: left to the workgroup-per-cell kernel
+    a.pn[k] = -1;
+    a.tcount[k] = 0;
+    atomicOr(&a.flags[3], 1);
     return;
   }
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1], wk = a.w[k];

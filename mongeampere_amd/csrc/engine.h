@@ -68,6 +68,15 @@ struct TriData {
 // of one Hessian row (neighbours + diagonal).
 constexpr int kMaxPoly = 64;
 constexpr int kMaxRow = 65;
+// Capacity of the one-workgroup-per-cell kernel for larger cells (polygon in
+// LDS), its workgroup size and the staging of one of its rows.
+constexpr int kHugePoly = 1024;
+constexpr int kHugeBlock = 256;
+constexpr int kHugeRow = kHugePoly + 1;
+// Cells whose bin rectangle spans more bins than this skip the one-thread-
+// per-cell kernels (their candidate loops are sequential) for the
+// one-workgroup-per-cell kernel.
+constexpr int kBigRect = 64;
 
 struct EvalBuffers {
   DBuf<double> w_s;          // weights in Morton order
@@ -94,6 +103,9 @@ struct EvalBuffers {
   DBuf<double> px, py, eacc, bbox;
   DBuf<float> pxf, pyf, eaccf;   // FP32 polygons and edge integrals (mixed mode)
   DBuf<int32_t> plab, ehit, pn, tcount, poff, pair_cell, pair_tri, glist;
+  // cells beyond the general kernel (list + count at [n]) and their long rows
+  DBuf<int32_t> hlist, hrow_col;
+  DBuf<double> hrow_val;
 };
 
 struct SolverBuffers {

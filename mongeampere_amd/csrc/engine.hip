@@ -203,15 +203,16 @@ __global__ void __launch_bounds__(kVB) k_cg_dir(int64_t n, int parity, const dou
 __global__ void k_noop() {}
 
 // Reductions for the Newton driver: min m, ||m - nu||^2, max |m-nu|/nu, nu.w,
-// argmin m.  g = m - nu is written.
+// sum(m - nu).  g = m - nu is written.
 __global__ void __launch_bounds__(kVB) k_newton_stats(int64_t n, const double *m,
                                                       const double *nu, const double *w,
                                                       double *g, double *part) {
   __shared__ double s[kVB];
-  double mn = HUGE_VAL, ss = 0.0, mr = 0.0, dot = 0.0;
+  double mn = HUGE_VAL, ss = 0.0, mr = 0.0, dot = 0.0, sg = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kVB + threadIdx.x; i < n; i += (int64_t)kVGrid * kVB) {
     const double gi = m[i] - nu[i];
     g[i] = gi;
+    sg += gi;
     mn = fmin(mn, m[i]);
     ss += gi * gi;
     mr = fmax(mr, fabs(gi) / nu[i]);
@@ -236,11 +237,13 @@ __global__ void __launch_bounds__(kVB) k_newton_stats(int64_t n, const double *m
   __syncthreads();
   ss = block_sum(ss, s);
   dot = block_sum(dot, s);
+  sg = block_sum(sg, s);
   if (threadIdx.x == 0) {
-    part[4 * blockIdx.x + 0] = mn;
-    part[4 * blockIdx.x + 1] = ss;
-    part[4 * blockIdx.x + 2] = mr;
-    part[4 * blockIdx.x + 3] = dot;
+    part[5 * blockIdx.x + 0] = mn;
+    part[5 * blockIdx.x + 1] = ss;
+    part[5 * blockIdx.x + 2] = mr;
+    part[5 * blockIdx.x + 3] = dot;
+    part[5 * blockIdx.x + 4] = sg;
   }
 }
 __global__ void k_axpy_to(int64_t n, const double *x, double a, const double *d, double *y) {
@@ -251,6 +254,10 @@ __global__ void k_neg(int64_t n, double *x) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) x[i] = -x[i];
 }
+__global__ void k_add_const(int64_t n, double *x, double c) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] += c;
+}
 
 __global__ void k_split2(const double *m2, int64_t n, int c, double *o) {
   int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -258,30 +265,32 @@ __global__ void k_split2(const double *m2, int64_t n, int c, double *o) {
 }
 
 struct NewtonStats {
-  double minm, l2, maxrel, nuw;
+  double minm, l2, maxrel, nuw, sumg;
 };
 
 static hipError_t newton_stats(ma_engine *e, const double *m, const double *nu, const double *w,
                                double *g, NewtonStats &st) {
   SolverBuffers &S = e->sb;
-  MA_HIP_RET(S.red.ensure(4 * kVGrid));
+  MA_HIP_RET(S.red.ensure(5 * kVGrid + 16));
   hipLaunchKernelGGL(k_newton_stats, dim3(kVGrid), dim3(kVB), 0, e->stream, e->n, m, nu, w, g,
                      S.red.p);
-  std::vector<double> h(4 * kVGrid);
-  MA_HIP_RET(hipMemcpyAsync(h.data(), S.red.p, sizeof(double) * 4 * kVGrid,
+  std::vector<double> h(5 * kVGrid);
+  MA_HIP_RET(hipMemcpyAsync(h.data(), S.red.p, sizeof(double) * 5 * kVGrid,
                             hipMemcpyDeviceToHost, e->stream));
   MA_HIP_RET(hipStreamSynchronize(e->stream));
   st.minm = HUGE_VAL;
   st.maxrel = 0.0;
-  double ss = 0.0, dot = 0.0;
+  double ss = 0.0, dot = 0.0, sg = 0.0;
   for (int b = 0; b < kVGrid; b++) {
-    st.minm = std::min(st.minm, h[4 * b]);
-    ss += h[4 * b + 1];
-    st.maxrel = std::max(st.maxrel, h[4 * b + 2]);
-    dot += h[4 * b + 3];
+    st.minm = std::min(st.minm, h[5 * b]);
+    ss += h[5 * b + 1];
+    st.maxrel = std::max(st.maxrel, h[5 * b + 2]);
+    dot += h[5 * b + 3];
+    sg += h[5 * b + 4];
   }
   st.l2 = std::sqrt(ss);
   st.nuw = dot;
+  st.sumg = sg;
   return hipSuccess;
 }
 
@@ -829,6 +838,13 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
     CsrView M;
     int32_t gi = 0;
     MA_HIP_TRY(eval_csr(e, M, gi));
+    // H is singular along the constant vector: the part of g along it (the
+    // rounding-level difference between the computed total mass and sum(nu))
+    // cannot be corrected by any step.  Removing it spreads that difference
+    // over all cells instead of leaving it in the gauge cell, where it would
+    // set a floor of |sum g| / nu_gauge on the max relative error.
+    hipLaunchKernelGGL(k_add_const, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream,
+                       n, S.g.p, -ns.sumg / (double)n);
     MA_HIP_TRY(pcg(e, M, gi, S.g.p, S.d.p, rel, o.cg_maxiter, o.preconditioner, &it, &rr));
     hipLaunchKernelGGL(k_neg, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, n, S.d.p);
     st.cg_iter += it;
@@ -891,7 +907,8 @@ ma_status ma_set_option(ma_engine *e, int option, int64_t value) {
   if (!e) return MA_BAD_INPUT;
   switch (option) {
     case MA_OPT_FORCE_GENERAL:
-      e->force_general = value != 0;
+      if (value < 0 || value > 2) return MA_BAD_INPUT;
+      e->force_general = (int)value;
       return MA_OK;
     case MA_OPT_CAP_CAND:
       if (value < 4 || value > 4096) return MA_BAD_INPUT;

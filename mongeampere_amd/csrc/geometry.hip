@@ -613,6 +613,78 @@ __device__ inline bool clip_tri_edge(const Poly &P, Poly &Q, double ax, double a
   return true;
 }
 
+// Integrals over one convex fragment (vertices (qx, qy)[0..m), coordinates
+// local to the Dirac) of triangle t, added to S: mass and int rho |x|^2
+// (MODE 0), or mass and first moments (MODE 1, 2); MODE 2 also returns in dv
+// the integrals of (|x|^2 - w_k) phi_v for the triangle's three hat functions.
+// Fan from vertex 0 (include/MA/quadrature.hpp:122-143); the mass by the
+// centroid rule, the rest by the Albrecht-Collatz points (exact for the
+// cubic integrands).
+struct FragSums {
+  double mass = 0.0, intg = 0.0, momx = 0.0, momy = 0.0;
+};
+template <int MODE>
+__device__ inline void frag_integrals(const double *qx, const double *qy, int m,
+                                      const TriData &t, double yx, double yy, double wk,
+                                      FragSums &S, double dv[3]) {
+  const double r0 = t.a * yx + t.b * yy + t.c;  // rho at the Dirac
+  double la[3], lb[3], lc[3];
+  if (MODE == 2) {
+    // barycentric coordinates of the triangle as affine functions
+    const double ax = t.x[0] - yx, ay = t.y[0] - yy, bx = t.x[1] - yx, by = t.y[1] - yy;
+    const double cx = t.x[2] - yx, cy = t.y[2] - yy;
+    const double inv = 1.0 / ((bx - ax) * (cy - ay) - (cx - ax) * (by - ay));
+    la[0] = (by - cy) * inv; lb[0] = (cx - bx) * inv; lc[0] = (bx * cy - cx * by) * inv;
+    la[1] = (cy - ay) * inv; lb[1] = (ax - cx) * inv; lc[1] = (cx * ay - ax * cy) * inv;
+    la[2] = (ay - by) * inv; lb[2] = (bx - ax) * inv; lc[2] = (ax * by - bx * ay) * inv;
+  }
+  for (int i = 1; i + 1 < m; i++) {
+    const double x0 = qx[0], y0 = qy[0], x1 = qx[i], y1 = qy[i];
+    const double x2 = qx[i + 1], y2 = qy[i + 1];
+    const double area = 0.5 * ((x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0));
+    const double gx = (x0 + x1 + x2) / 3.0, gy = (y0 + y1 + y2) / 3.0;
+    S.mass += area * (r0 + t.a * gx + t.b * gy);
+    AC6 qp;
+    ac6_points(x0, y0, x1, y1, x2, y2, qp);
+    double s1 = 0.0, s9 = 0.0, m1x = 0.0, m9x = 0.0, m1y = 0.0, m9y = 0.0;
+    double c1[3] = {0.0, 0.0, 0.0}, c9[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < 6; p++) {
+      const double rp = r0 + t.a * qp.px[p] + t.b * qp.py[p];
+      if (MODE == 2) {
+        const double c = qp.px[p] * qp.px[p] + qp.py[p] * qp.py[p] - wk;
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+          const double f = c * (la[v] * qp.px[p] + lb[v] * qp.py[p] + lc[v]);
+          if (p < 3) c1[v] += f; else c9[v] += f;
+        }
+      }
+      if (MODE == 0) {
+        const double f = rp * (qp.px[p] * qp.px[p] + qp.py[p] * qp.py[p]);
+        if (p < 3) s1 += f; else s9 += f;
+      } else {
+        if (p < 3) {
+          m1x += rp * qp.px[p];
+          m1y += rp * qp.py[p];
+        } else {
+          m9x += rp * qp.px[p];
+          m9y += rp * qp.py[p];
+        }
+      }
+    }
+    if (MODE == 0) {
+      S.intg += area * (s1 * (1.0 / 30.0) + s9 * (9.0 / 30.0));
+    } else {
+      S.momx += area * (m1x * (1.0 / 30.0) + m9x * (9.0 / 30.0));
+      S.momy += area * (m1y * (1.0 / 30.0) + m9y * (9.0 / 30.0));
+    }
+    if (MODE == 2) {
+#pragma unroll
+      for (int v = 0; v < 3; v++) dv[v] += area * (c1[v] * (1.0 / 30.0) + c9[v] * (9.0 / 30.0));
+    }
+  }
+}
+
 // General per-Dirac kernel (polygons up to kMaxPoly vertices in private
 // memory) for the cells listed in `list`: the rare cells that overflow the
 // register/LDS fast path.
@@ -627,12 +699,16 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
     const int32_t *__restrict__ tbin_start, const int32_t *__restrict__ tbin_items,
     const TriData *__restrict__ tri, double *mass_out, double *intg_out, double *fval_part,
     double *mom_out, int32_t *row_cnt, int32_t *row_col, double *row_val, int cap_row,
-    const int32_t *__restrict__ triv, double *dval, int32_t *flags,
-    unsigned long long *counters) {
+    const int32_t *__restrict__ triv, double *dval, int32_t *hlist, int force_huge,
+    int32_t *flags, unsigned long long *counters) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= *nlist) return;
   const int64_t k = list[idx];
   const int rx0 = rect[4 * k], ry0 = rect[4 * k + 1], rx1 = rect[4 * k + 2], ry1 = rect[4 * k + 3];
+  if (rx0 <= rx1 && (int64_t)(rx1 - rx0 + 1) * (ry1 - ry0 + 1) > kBigRect) {
+    hlist[atomicAdd(&hlist[n], 1)] = (int32_t)k;  // k_cell_huge
+    return;
+  }
   const double yx = pos[2 * k], yy = pos[2 * k + 1], wk = w[k];
   Poly P, Q, F;
   P.n = 0;
@@ -682,8 +758,14 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
         for (int v = 0; v < P.n; v++) rho2 = fmax(rho2, P.x[v] * P.x[v] + P.y[v] * P.y[v]);
       }
     }
+  // cells beyond this kernel's capacity go to k_cell_huge (a fragment of an
+  // n-gon has at most n + 3 vertices, a row at most n + 1 entries)
+  if (overflow || P.n > kMaxPoly - 4 || force_huge) {
+    hlist[atomicAdd(&hlist[n], 1)] = (int32_t)k;
+    return;
+  }
 
-  double mass = 0.0, intg = 0.0, momx = 0.0, momy = 0.0;
+  FragSums S;
   double acc[kMaxPoly];
   uint64_t touched = 0u;
   int nfrag = 0;
@@ -751,60 +833,8 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
           }
           nfrag++;
           const double r0 = t.a * yx + t.b * yy + t.c;  // rho at the Dirac
-          double la[3], lb[3], lc[3], dv[3] = {0.0, 0.0, 0.0};
-          if (MODE == 2) {
-            const double inv = 1.0 / ((bxx - ax) * (cy - ay) - (cx - ax) * (byy - ay));
-            la[0] = (byy - cy) * inv; lb[0] = (cx - bxx) * inv; lc[0] = (bxx * cy - cx * byy) * inv;
-            la[1] = (cy - ay) * inv;  lb[1] = (ax - cx) * inv;  lc[1] = (cx * ay - ax * cy) * inv;
-            la[2] = (ay - byy) * inv; lb[2] = (bxx - ax) * inv; lc[2] = (ax * byy - bxx * ay) * inv;
-          }
-          // fan from vertex 0 (include/MA/quadrature.hpp:122-143)
-          for (int i = 1; i + 1 < m; i++) {
-            const double x0 = Q.x[0], y0 = Q.y[0], x1 = Q.x[i], y1 = Q.y[i];
-            const double x2 = Q.x[i + 1], y2 = Q.y[i + 1];
-            const double area = 0.5 * ((x1 - x0) * (y2 - y0) - (x2 - x0) * (y1 - y0));
-            const double gx = (x0 + x1 + x2) / 3.0, gy = (y0 + y1 + y2) / 3.0;
-            mass += area * (r0 + t.a * gx + t.b * gy);
-            AC6 qp;
-            ac6_points(x0, y0, x1, y1, x2, y2, qp);
-            double s1 = 0.0, s9 = 0.0, m1x = 0.0, m9x = 0.0, m1y = 0.0, m9y = 0.0;
-            double c1[3] = {0.0, 0.0, 0.0}, c9[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int p = 0; p < 6; p++) {
-              const double rp = r0 + t.a * qp.px[p] + t.b * qp.py[p];
-              if (MODE == 2) {
-                const double c = qp.px[p] * qp.px[p] + qp.py[p] * qp.py[p] - wk;
-#pragma unroll
-                for (int v = 0; v < 3; v++) {
-                  const double f = c * (la[v] * qp.px[p] + lb[v] * qp.py[p] + lc[v]);
-                  if (p < 3) c1[v] += f; else c9[v] += f;
-                }
-              }
-              if (MODE == 0) {
-                const double f = rp * (qp.px[p] * qp.px[p] + qp.py[p] * qp.py[p]);
-                if (p < 3) s1 += f; else s9 += f;
-              } else {
-                if (p < 3) {
-                  m1x += rp * qp.px[p];
-                  m1y += rp * qp.py[p];
-                } else {
-                  m9x += rp * qp.px[p];
-                  m9y += rp * qp.py[p];
-                }
-              }
-            }
-            if (MODE == 0) {
-              intg += area * (s1 * (1.0 / 30.0) + s9 * (9.0 / 30.0));
-            } else {
-              momx += area * (m1x * (1.0 / 30.0) + m9x * (9.0 / 30.0));
-              momy += area * (m1y * (1.0 / 30.0) + m9y * (9.0 / 30.0));
-            }
-            if (MODE == 2) {
-#pragma unroll
-              for (int v = 0; v < 3; v++)
-                dv[v] += area * (c1[v] * (1.0 / 30.0) + c9[v] * (9.0 / 30.0));
-            }
-          }
+          double dv[3] = {0.0, 0.0, 0.0};
+          frag_integrals<MODE>(Q.x, Q.y, m, t, yx, yy, wk, S, dv);
           if (MODE == 2) {
             const int64_t ti = tbin_items[q];
 #pragma unroll
@@ -828,9 +858,9 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
       }
   }
   if (MODE == 0) {
-    mass_out[k] = mass;
-    intg_out[k] = intg;
-    fval_part[k] = wk * mass - intg;
+    mass_out[k] = S.mass;
+    intg_out[k] = S.intg;
+    fval_part[k] = wk * S.mass - S.intg;
     // row: off-diagonals sorted by column, duplicates summed, diagonal last-in
     int32_t cols[kMaxPoly + 1];
     double vals[kMaxPoly + 1];
@@ -881,27 +911,556 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
     }
     row_cnt[k] = cnt;
   } else {
-    mass_out[k] = mass;
+    mass_out[k] = S.mass;
     // mode 2 keeps the moments about the Dirac
-    mom_out[2 * k] = momx + (MODE == 1 ? yx * mass : 0.0);
-    mom_out[2 * k + 1] = momy + (MODE == 1 ? yy * mass : 0.0);
+    mom_out[2 * k] = S.momx + (MODE == 1 ? yx * S.mass : 0.0);
+    mom_out[2 * k + 1] = S.momy + (MODE == 1 ? yy * S.mass : 0.0);
   }
   if (overflow) atomicOr(&flags[0], 1);
   atomicAdd(&counters[0], (unsigned long long)nfrag);
   atomicMax(&counters[2], (unsigned long long)np);
 }
 
+// ---------------------------------------------------------------------------
+// Cells beyond the general kernel (more than kMaxPoly - 4 vertices, or a bin
+// rectangle of more than kBigRect bins: e.g. a cell in a sparse region
+// bordering a dense cluster, or far displaced from its Dirac), one workgroup
+// per cell.  The polygon (LDS, up to kHugePoly vertices) is clipped by the
+// owners listed in the rectangle's bins, batch by batch, deepest cut first,
+// each clip a parallel Sutherland-Hodgman step: every thread classifies four consecutive vertices, emits 0..2 output
+// vertices for each, and a block prefix sum places them.  Fragments are the
+// triangles of the cell's bins clipped by the cell's edge lines, one
+// triangle per thread; Hessian edge integrals are summed in LDS.
+// ---------------------------------------------------------------------------
+constexpr int kHugeChunk = kHugePoly / kHugeBlock;  // vertices per thread
+constexpr int kFragCap = 48;                         // fragment vertices per thread
+constexpr int kHugeMaxClips = 1 << 16;               // clips of one cell
+
+static_assert(kHugeChunk * kHugeBlock == kHugePoly, "chunking");
+
+// Exclusive prefix sum over the block (kHugeBlock threads); *total gets the sum.
+__device__ inline int huge_scan(int v, int *tmp, int *total) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) tmp[wid] = x;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int i = 0; i < kHugeBlock / 64; i++) {
+    off += (i < wid) ? tmp[i] : 0;
+    tot += tmp[i];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + x - v;
+}
+
+// Block-wide sum (op 0) or max (op 1) of a double.
+template <int OP>
+__device__ inline double huge_reduce(double v, double *tmp) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const double y = __shfl_xor(v, o, 64);
+    v = OP == 0 ? v + y : fmax(v, y);
+  }
+  if ((threadIdx.x & 63) == 0) tmp[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = tmp[0];
+  for (int i = 1; i < kHugeBlock / 64; i++) r = OP == 0 ? r + tmp[i] : fmax(r, tmp[i]);
+  __syncthreads();
+  return r;
+}
+
+// Clip of a thread's fragment (triangle piece, FP64 local coordinates) by the
+// closed half-plane N.(X - A) >= 0 of cell edge `label`; points within
+// rounding of the line count as on it.  Returns false on capacity overflow.
+__device__ inline bool clip_frag(int &m, double *fx, double *fy, int *fl, double *gx, double *gy,
+                                 int *gl, double Ax, double Ay, double Nx, double Ny, int label) {
+  double s[kFragCap];
+  int nin = 0;
+  const double nn = fabs(Nx) + fabs(Ny);
+  for (int i = 0; i < m; i++) {
+    const double dx = fx[i] - Ax, dy = fy[i] - Ay;
+    s[i] = Nx * dx + Ny * dy;
+    if (fabs(s[i]) <= kSnap * nn * (fabs(fx[i]) + fabs(fy[i]) + fabs(dx) + fabs(dy))) s[i] = 0.0;
+    nin += s[i] >= 0.0;
+  }
+  // a fragment edge lying on the line (e.g. a triangle edge along the cell
+  // edge) is a piece of the cell edge
+  for (int i = 0; i < m; i++)
+    if (s[i] == 0.0 && s[(i + 1 == m) ? 0 : i + 1] == 0.0) fl[i] = label;
+  if (nin == m) return true;
+  if (nin == 0) {
+    m = 0;
+    return true;
+  }
+  int q = 0;
+  for (int i = 0; i < m; i++) {
+    const int i1 = (i + 1 == m) ? 0 : i + 1;
+    if (q + 2 > kFragCap) return false;
+    if (s[i] >= 0.0) {
+      gx[q] = fx[i];
+      gy[q] = fy[i];
+      gl[q++] = fl[i];
+      if (s[i1] < 0.0) {
+        const double t = s[i] / (s[i] - s[i1]);
+        gx[q] = fx[i] + t * (fx[i1] - fx[i]);
+        gy[q] = fy[i] + t * (fy[i1] - fy[i]);
+        gl[q++] = label;
+      }
+    } else if (s[i1] >= 0.0) {
+      const double t = s[i] / (s[i] - s[i1]);
+      gx[q] = fx[i] + t * (fx[i1] - fx[i]);
+      gy[q] = fy[i] + t * (fy[i1] - fy[i]);
+      gl[q++] = fl[i];
+    }
+  }
+  for (int i = 0; i < q; i++) {
+    fx[i] = gx[i];
+    fy[i] = gy[i];
+    fl[i] = gl[i];
+  }
+  m = q;
+  return true;
+}
+
+// Edge labels of the bin-rectangle sides inside k_cell_huge (all negative,
+// so never a neighbour): bottom, right, top, left.
+constexpr int kHugeSide0 = -2;
+
+template <int MODE>
+__global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
+    const int32_t *__restrict__ hlist, Grid g, int64_t n, const double *__restrict__ pos,
+    const double *__restrict__ w, const int32_t *__restrict__ perm,
+    const int32_t *__restrict__ rect, const int32_t *__restrict__ cand_start,
+    const int32_t *__restrict__ cand, const int32_t *__restrict__ tbin_start,
+    const int32_t *__restrict__ tbin_items, const TriData *__restrict__ tri, double *mass_out,
+    double *intg_out, double *fval_part, double *mom_out, int32_t *row_cnt, int32_t *row_col,
+    double *row_val, int cap_row, int32_t *hrow_col, double *hrow_val,
+    const int32_t *__restrict__ triv, double *dval, int32_t *flags,
+    unsigned long long *counters) {
+  __shared__ double sx[2][kHugePoly], sy[2][kHugePoly];
+  __shared__ int sl[2][kHugePoly];
+  __shared__ double sacc[kHugePoly];  // clip values while building, edge integrals after
+  __shared__ unsigned char stouch[kHugePoly];
+  __shared__ int itmp[kHugeBlock / 64];
+  __shared__ double dtmp[kHugeBlock / 64];
+  __shared__ int s_empty, s_best;
+  const int tid = threadIdx.x;
+  const int slot = blockIdx.x;
+  const int64_t k = hlist[slot];
+  const int rx0 = rect[4 * k], ry0 = rect[4 * k + 1], rx1 = rect[4 * k + 2], ry1 = rect[4 * k + 3];
+  const double yx = pos[2 * k], yy = pos[2 * k + 1], wk = w[k];
+  int np = 0, cur = 0;
+  bool fail = false;
+  double X0 = 0.0, X1 = 0.0, Y0 = 0.0, Y1 = 0.0;
+  if (rx0 <= rx1) {
+    X0 = g.x0 + rx0 * g.h - yx;
+    X1 = g.x0 + (rx1 + 1) * g.h - yx;
+    Y0 = g.y0 + ry0 * g.h - yy;
+    Y1 = g.y0 + (ry1 + 1) * g.h - yy;
+    if (tid == 0) {
+      sx[0][0] = X0; sy[0][0] = Y0; sl[0][0] = kHugeSide0;
+      sx[0][1] = X1; sy[0][1] = Y0; sl[0][1] = kHugeSide0 - 1;
+      sx[0][2] = X1; sy[0][2] = Y1; sl[0][2] = kHugeSide0 - 2;
+      sx[0][3] = X0; sy[0][3] = Y1; sl[0][3] = kHugeSide0 - 3;
+    }
+    np = 4;
+  }
+  if (tid == 0) s_empty = 0;
+  __syncthreads();
+  // Bins of the rectangle in Chebyshev rings around its centre, kHugeBlock
+  // bins (one per thread) at a time; bins outside the bounding box of the
+  // current polygon are skipped (the cell lies inside the polygon, so their
+  // owners cannot border it).  Within a batch the polygon is clipped by the
+  // candidate cutting deepest until no candidate cuts by more than rounding;
+  // a half-plane already applied cannot cut again, so candidates listed in
+  // several bins need no deduplication.
+  const int v0 = tid * kHugeChunk;
+  int nclip = 0;  // bound on the clips of one cell (every loop must end)
+  const int cbx = (rx0 + rx1) >> 1, cby = (ry0 + ry1) >> 1;
+  int qb[4] = {rx0, ry0, rx1, ry1};  // bin box of the polygon
+  for (int r = 0; np > 0 && !fail; r++) {
+    const int reach = max(max(cbx - qb[0], qb[2] - cbx), max(cby - qb[1], qb[3] - cby));
+    if (r > reach) break;
+    const int len = r == 0 ? 1 : 8 * r;  // ring positions, counter-clockwise from the corner
+    for (int p0 = 0; p0 < len && np > 0 && !fail; p0 += kHugeBlock) {
+      int32_t c0 = 0, c1 = 0;  // this thread's bin: candidates [c0, c1)
+      const int pos_i = p0 + tid;
+      if (pos_i < len) {
+        int bx = cbx, by = cby;
+        if (r > 0) {
+          const int side = pos_i / (2 * r), o = pos_i % (2 * r);
+          if (side == 0) { bx = cbx - r + o; by = cby - r; }
+          else if (side == 1) { bx = cbx + r; by = cby - r + o; }
+          else if (side == 2) { bx = cbx + r - o; by = cby + r; }
+          else { bx = cbx - r; by = cby + r - o; }
+        }
+        if (bx >= qb[0] && bx <= qb[2] && by >= qb[1] && by <= qb[3]) {
+          const int64_t b = morton2((uint32_t)bx, (uint32_t)by);
+          c0 = cand_start[b];
+          c1 = cand_start[b + 1];
+        }
+      }
+      while (true) {
+        if (tid == 0) s_best = INT_MAX;
+        double best = kInf;
+        int32_t bj = -1;
+        for (int32_t i = c0; i < c1; i++) {
+          const int32_t j = cand[i];
+          if (j == k) continue;
+          const double Yx = pos[2 * j] - yx, Yy = pos[2 * j + 1] - yy, wj = w[j];
+          const double d2 = Yx * Yx + Yy * Yy;
+          if (d2 == 0.0) {
+            // coincident Diracs: the larger weight (then the smaller caller
+            // index) hides the other
+            if (wj > wk || (wj == wk && perm[j] < perm[k])) s_empty = 1;
+            continue;
+          }
+          const double kk = d2 + wk - wj;
+          double dmin = kInf;
+          for (int v = 0; v < np; v++) {
+            const double tx = sx[cur][v] * Yx, ty = sy[cur][v] * Yy;
+            const double sv = kk - 2.0 * (tx + ty);
+            if (sv < -1e-14 * (fabs(kk) + 2.0 * (fabs(tx) + fabs(ty)))) dmin = fmin(dmin, sv);
+          }
+          const double depth = dmin / sqrt(d2);
+          if (depth < best) {
+            best = depth;
+            bj = j;
+          }
+        }
+        // deepest over the block; the smallest Dirac index among equals
+        const double gbest = -huge_reduce<1>(-best, dtmp);
+        if (bj >= 0 && best == gbest) atomicMin(&s_best, bj);
+        __syncthreads();
+        const int32_t j = s_best;
+        if (s_empty) {
+          np = 0;
+          break;
+        }
+        if (j == INT_MAX) break;
+        if (++nclip > kHugeMaxClips) {
+          fail = true;
+          break;
+        }
+        const double Yx = pos[2 * j] - yx, Yy = pos[2 * j + 1] - yy, wj = w[j];
+        const double kk = Yx * Yx + Yy * Yy + wk - wj;
+        // clip values; those within rounding of zero are snapped onto the
+        // line, so that a line through a vertex keeps that vertex instead of
+        // adding a coincident copy (the region differs by measure zero)
+        for (int c = 0; c < kHugeChunk; c++) {
+          const int v = v0 + c;
+          if (v < np) {
+            const double tx = sx[cur][v] * Yx, ty = sy[cur][v] * Yy;
+            const double sv = kk - 2.0 * (tx + ty);
+            sacc[v] = (fabs(sv) <= 1e-14 * (fabs(kk) + 2.0 * (fabs(tx) + fabs(ty)))) ? 0.0 : sv;
+          }
+        }
+        __syncthreads();
+        // per vertex v and edge v -> v1: s_v > 0 keeps v (and adds the exit
+        // point when s_v1 < 0); s_v = 0 keeps v, which starts the new edge
+        // when s_v1 < 0; s_v < 0 adds the entry point when s_v1 > 0
+        int emit[kHugeChunk], cnt = 0;
+        for (int c = 0; c < kHugeChunk; c++) {
+          const int v = v0 + c;
+          emit[c] = 0;
+          if (v < np) {
+            const int v1 = (v + 1 == np) ? 0 : v + 1;
+            const double s0 = sacc[v], s1 = sacc[v1];
+            emit[c] = s0 > 0.0 ? (s1 < 0.0 ? 2 : 1) : s0 == 0.0 ? 1 : (s1 > 0.0 ? 1 : 0);
+          }
+          cnt += emit[c];
+        }
+        int total;
+        int o = huge_scan(cnt, itmp, &total);
+        if (total > kHugePoly) {
+          fail = true;
+          break;
+        }
+        const int nx = cur ^ 1;
+        for (int c = 0; c < kHugeChunk; c++) {
+          const int v = v0 + c;
+          if (emit[c] == 0) continue;
+          const int v1 = (v + 1 == np) ? 0 : v + 1;
+          const double s0 = sacc[v], s1 = sacc[v1];
+          if (s0 >= 0.0) {
+            sx[nx][o] = sx[cur][v];
+            sy[nx][o] = sy[cur][v];
+            sl[nx][o++] = (s0 == 0.0 && s1 < 0.0) ? j : sl[cur][v];
+            if (s0 > 0.0 && s1 < 0.0) {  // exit point: the new edge starts here
+              const double t = s0 / (s0 - s1);
+              sx[nx][o] = sx[cur][v] + t * (sx[cur][v1] - sx[cur][v]);
+              sy[nx][o] = sy[cur][v] + t * (sy[cur][v1] - sy[cur][v]);
+              sl[nx][o++] = j;
+            }
+          } else {  // entry point on edge v
+            const double t = s0 / (s0 - s1);
+            sx[nx][o] = sx[cur][v] + t * (sx[cur][v1] - sx[cur][v]);
+            sy[nx][o] = sy[cur][v] + t * (sy[cur][v1] - sy[cur][v]);
+            sl[nx][o++] = sl[cur][v];
+          }
+        }
+        __syncthreads();
+        cur = nx;
+        np = total < 3 ? 0 : total;  // a point or a segment: no area
+        if (np == 0) break;
+      }
+      // shrink the bin box to the polygon's
+      if (np > 0 && !fail) {
+        double ex0 = kInf, ey0 = kInf, ex1 = -kInf, ey1 = -kInf;
+        for (int c = 0; c < kHugeChunk; c++) {
+          const int v = v0 + c;
+          if (v >= np) continue;
+          ex0 = fmin(ex0, sx[cur][v]);
+          ex1 = fmax(ex1, sx[cur][v]);
+          ey0 = fmin(ey0, sy[cur][v]);
+          ey1 = fmax(ey1, sy[cur][v]);
+        }
+        ex0 = -huge_reduce<1>(-ex0, dtmp);
+        ey0 = -huge_reduce<1>(-ey0, dtmp);
+        ex1 = huge_reduce<1>(ex1, dtmp);
+        ey1 = huge_reduce<1>(ey1, dtmp);
+        qb[0] = max(qb[0], bin_coord(ex0 + yx, g.x0, g.h, g.G));
+        qb[1] = max(qb[1], bin_coord(ey0 + yy, g.y0, g.h, g.G));
+        qb[2] = min(qb[2], bin_coord(ex1 + yx, g.x0, g.h, g.G));
+        qb[3] = min(qb[3], bin_coord(ey1 + yy, g.y0, g.h, g.G));
+      }
+    }
+  }
+  if (fail) {
+    if (tid == 0) {
+      atomicOr(&flags[0], 1);
+      counters[7] = ((unsigned long long)(uint32_t)k << 32) | ((unsigned long long)np << 16) |
+                    (unsigned long long)min(nclip, 0xffff);
+    }
+    return;
+  }
+  // edge lines (a point A on the edge and an inward normal N) into the
+  // spare buffer; edge integrals start at zero
+  const int nb_ = cur ^ 1;
+  double bx0 = kInf, by0 = kInf, bx1 = -kInf, by1 = -kInf;
+  for (int c = 0; c < kHugeChunk; c++) {
+    const int v = v0 + c;
+    if (v >= np) continue;
+    const int lab = sl[cur][v];
+    double Nx, Ny;
+    if (lab >= 0) {
+      Nx = yx - pos[2 * lab];
+      Ny = yy - pos[2 * lab + 1];
+    } else {
+      const int side = kHugeSide0 - lab;  // 0 bottom, 1 right, 2 top, 3 left
+      Nx = side == 1 ? -1.0 : side == 3 ? 1.0 : 0.0;
+      Ny = side == 0 ? 1.0 : side == 2 ? -1.0 : 0.0;
+    }
+    sx[nb_][v] = Nx;
+    sy[nb_][v] = Ny;
+    sacc[v] = 0.0;
+    stouch[v] = 0;
+    bx0 = fmin(bx0, sx[cur][v]);
+    bx1 = fmax(bx1, sx[cur][v]);
+    by0 = fmin(by0, sy[cur][v]);
+    by1 = fmax(by1, sy[cur][v]);
+  }
+  bx0 = -huge_reduce<1>(-bx0, dtmp) + yx;
+  by0 = -huge_reduce<1>(-by0, dtmp) + yy;
+  bx1 = huge_reduce<1>(bx1, dtmp) + yx;
+  by1 = huge_reduce<1>(by1, dtmp) + yy;
+  FragSums S;
+  int nfrag = 0;
+  bool overflow = false;
+  if (np > 0) {
+    const int qx0 = max(bin_coord(bx0, g.x0, g.h, g.G), rx0);
+    const int qx1 = min(bin_coord(bx1, g.x0, g.h, g.G), rx1);
+    const int qy0 = max(bin_coord(by0, g.y0, g.h, g.G), ry0);
+    const int qy1 = min(bin_coord(by1, g.y0, g.h, g.G), ry1);
+    const int wx = qx1 - qx0 + 1;
+    const int64_t nbins = (qx0 <= qx1 && qy0 <= qy1) ? (int64_t)wx * (qy1 - qy0 + 1) : 0;
+    double fx[kFragCap], fy[kFragCap], gx[kFragCap], gy[kFragCap];
+    int fl[kFragCap], gl[kFragCap];
+    for (int64_t ib = tid; ib < nbins; ib += kHugeBlock) {
+      const int bx = qx0 + (int)(ib % wx), by = qy0 + (int)(ib / wx);
+      const int64_t b = morton2((uint32_t)bx, (uint32_t)by);
+      for (int32_t q = tbin_start[b]; q < tbin_start[b + 1]; q++) {
+        const TriData t = tri[tbin_items[q]];
+        double tx0, ty0, tx1, ty1;
+        tri_bbox(t, tx0, ty0, tx1, ty1);
+        if (tx1 < bx0 || tx0 > bx1 || ty1 < by0 || ty0 > by1) continue;
+        // each triangle once: in the bin of the lower-left corner of the
+        // overlap of the two bounding boxes
+        const int ox = max(bin_coord(fmax(tx0, bx0), g.x0, g.h, g.G), qx0);
+        const int oy = max(bin_coord(fmax(ty0, by0), g.y0, g.h, g.G), qy0);
+        if (ox != bx || oy != by) continue;
+        int m = 3;
+        for (int c = 0; c < 3; c++) {
+          fx[c] = t.x[c] - yx;
+          fy[c] = t.y[c] - yy;
+          fl[c] = -1;  // triangle edge
+        }
+        for (int v = 0; v < np && m > 0; v++)
+          if (!clip_frag(m, fx, fy, fl, gx, gy, gl, sx[cur][v], sy[cur][v], sx[nb_][v],
+                         sy[nb_][v], v)) {
+            overflow = true;
+            m = 0;
+          }
+        if (m < 3) continue;
+        {
+          double ar2 = 0.0, xlo = kInf, xhi = -kInf, ylo = kInf, yhi = -kInf, cm = 0.0;
+          for (int i = 0; i < m; i++) {
+            const int i1 = (i + 1 == m) ? 0 : i + 1;
+            ar2 += fx[i] * fy[i1] - fx[i1] * fy[i];
+            xlo = fmin(xlo, fx[i]);
+            xhi = fmax(xhi, fx[i]);
+            ylo = fmin(ylo, fy[i]);
+            yhi = fmax(yhi, fy[i]);
+            cm = fmax(cm, fmax(fabs(fx[i]), fabs(fy[i])));
+          }
+          if (!(0.5 * ar2 > 1e-13 * ((xhi - xlo) + (yhi - ylo)) * cm)) continue;
+        }
+        nfrag++;
+        double dv[3] = {0.0, 0.0, 0.0};
+        frag_integrals<MODE>(fx, fy, m, t, yx, yy, wk, S, dv);
+        if (MODE == 2) {
+          const int64_t ti = tbin_items[q];
+          for (int c = 0; c < 3; c++) atomicAdd(&dval[triv[3 * ti + c]], dv[c]);
+        }
+        if (MODE == 0) {
+          const double r0 = t.a * yx + t.b * yy + t.c;
+          for (int i = 0; i < m; i++) {
+            const int lab = fl[i];
+            if (lab < 0 || sl[cur][lab] < 0) continue;
+            const int i1 = (i + 1 == m) ? 0 : i + 1;
+            const double ex = fx[i1] - fx[i], ey = fy[i1] - fy[i];
+            if (ex == 0.0 && ey == 0.0) continue;
+            const double mx = 0.5 * (fx[i] + fx[i1]), my = 0.5 * (fy[i] + fy[i1]);
+            atomicAdd(&sacc[lab], sqrt(ex * ex + ey * ey) * (r0 + t.a * mx + t.b * my));
+            stouch[lab] = 1;
+          }
+        }
+      }
+    }
+  }
+  const double mass = huge_reduce<0>(S.mass, dtmp);
+  const double intg = huge_reduce<0>(S.intg, dtmp);
+  const double momx = huge_reduce<0>(S.momx, dtmp);
+  const double momy = huge_reduce<0>(S.momy, dtmp);
+  const int nfr = (int)huge_reduce<0>((double)nfrag, dtmp);
+  if (__syncthreads_or(overflow)) {
+    if (tid == 0) atomicOr(&flags[0], 2);  // fragment capacity
+    return;
+  }
+  if (MODE == 0) {
+    // row: neighbour edges that carry a fragment edge; repeated labels
+    // (degenerate edges) are merged into their first occurrence, entries
+    // ranked by column, the diagonal inserted in order
+    int keep[kHugeChunk];
+    double val[kHugeChunk];
+    double dsum = 0.0;
+    for (int c = 0; c < kHugeChunk; c++) {
+      const int v = v0 + c;
+      keep[c] = 0;
+      val[c] = 0.0;
+      if (v >= np || !stouch[v] || sl[cur][v] < 0) continue;
+      const int32_t j = sl[cur][v];
+      const double dx = yx - pos[2 * j], dy = yy - pos[2 * j + 1];
+      const double e = sacc[v] / (2.0 * sqrt(dx * dx + dy * dy));
+      dsum += e;
+      bool first = true;
+      for (int u = 0; u < v && first; u++) first = !(stouch[u] && sl[cur][u] == j);
+      keep[c] = first;
+    }
+    __syncthreads();
+    // merged values: sum over the repeated labels
+    for (int c = 0; c < kHugeChunk; c++) {
+      const int v = v0 + c;
+      if (!keep[c]) continue;
+      const int32_t j = sl[cur][v];
+      const double dx = yx - pos[2 * j], dy = yy - pos[2 * j + 1];
+      const double inv = 1.0 / (2.0 * sqrt(dx * dx + dy * dy));
+      double s = 0.0;
+      for (int u = v; u < np; u++)
+        if (stouch[u] && sl[cur][u] == j) s += sacc[u] * inv;
+      val[c] = -s;
+    }
+    const double diag = huge_reduce<0>(dsum, dtmp);
+    // kept flags to LDS (stouch is no longer needed) for the ranks
+    for (int c = 0; c < kHugeChunk; c++) {
+      const int v = v0 + c;
+      if (v < np) stouch[v] = (unsigned char)keep[c];
+    }
+    __syncthreads();
+    int nk = 0;
+    for (int c = 0; c < kHugeChunk; c++) nk += keep[c];
+    int nkept;
+    (void)huge_scan(nk, itmp, &nkept);
+    const int cnt = nkept > 0 ? nkept + 1 : 0;
+    int32_t *oc = row_col + k * (int64_t)cap_row;
+    double *ov = row_val + k * (int64_t)cap_row;
+    if (cnt > cap_row) {
+      // long row: staged at its slot; the first regular entry points there
+      if (tid == 0) oc[0] = slot;
+      oc = hrow_col + (int64_t)slot * kHugeRow;
+      ov = hrow_val + (int64_t)slot * kHugeRow;
+    }
+    for (int c = 0; c < kHugeChunk; c++) {
+      if (!keep[c]) continue;
+      const int32_t j = sl[cur][v0 + c];
+      int rank = 0;
+      for (int u = 0; u < np; u++) rank += (stouch[u] && sl[cur][u] < j);
+      const int p = rank + (j > k ? 1 : 0);
+      oc[p] = j;
+      ov[p] = val[c];
+    }
+    if (cnt > 0) {
+      int below = 0;
+      for (int c = 0; c < kHugeChunk; c++) below += keep[c] && sl[cur][v0 + c] < k;
+      int nbelow;
+      (void)huge_scan(below, itmp, &nbelow);
+      if (tid == 0) {
+        oc[nbelow] = (int32_t)k;
+        ov[nbelow] = diag;
+      }
+    }
+    if (tid == 0) {
+      row_cnt[k] = cnt;
+      mass_out[k] = mass;
+      intg_out[k] = intg;
+      fval_part[k] = wk * mass - intg;
+    }
+  } else if (tid == 0) {
+    mass_out[k] = mass;
+    mom_out[2 * k] = momx + (MODE == 1 ? yx * mass : 0.0);
+    mom_out[2 * k + 1] = momy + (MODE == 1 ? yy * mass : 0.0);
+  }
+  if (tid == 0) {
+    atomicAdd(&counters[0], (unsigned long long)nfr);
+    atomicMax(&counters[2], (unsigned long long)np);
+  }
+}
+
+// Rows longer than cap_row (from k_cell_huge) are read from the long-row
+// staging at the slot stored in their first regular entry.
 __global__ void k_csr_fill(int64_t n, const int32_t *row_cnt, const int32_t *row_col,
-                           const double *row_val, int cap_row, const int32_t *ptr, int32_t *col,
+                           const double *row_val, int cap_row, const int32_t *hrow_col,
+                           const double *hrow_val, const int32_t *ptr, int32_t *col,
                            double *val, double *diag) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int c = row_cnt[k];
   const int32_t o = ptr[k];
+  const int32_t *rc = row_col + k * (int64_t)cap_row;
+  const double *rv = row_val + k * (int64_t)cap_row;
+  if (c > cap_row) {
+    const int64_t s = rc[0];
+    rc = hrow_col + s * kHugeRow;
+    rv = hrow_val + s * kHugeRow;
+  }
   double d = 0.0;
   for (int i = 0; i < c; i++) {
-    const int32_t j = row_col[k * (int64_t)cap_row + i];
-    const double v = row_val[k * (int64_t)cap_row + i];
+    const int32_t j = rc[i];
+    const double v = rv[i];
     col[o + i] = j;
     val[o + i] = v;
     if (j == k) d = v;
@@ -1026,15 +1585,35 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   int32_t npairs = 0, ngen = 0;
   ma_status cs = run_cells(e, mode, fp32, &npairs, &ngen);
   if (cs != MA_OK) return cs;
+  int32_t nhuge = 0;
   if (ngen > 0) {
     // cells that overflowed the fast path
+    MA_HIP_TRY(B.hlist.ensure((size_t)n + 1));
+    MA_HIP_TRY(hipMemsetAsync(B.hlist.p + n, 0, sizeof(int32_t), e->stream));
     auto kern = mode == 0 ? k_cells_general<0> : mode == 1 ? k_cells_general<1>
                                                             : k_cells_general<2>;
     hipLaunchKernelGGL(kern, dim3(nblocks(ngen)), dim3(kBlock), 0, e->stream, B.glist.p,
                        B.glist.p + n, g, n, e->pos.p, B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p,
                        B.cand.p, e->tbin_start.p, e->tbin_items.p, e->tri.p,
                        B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p, B.row_cnt.p,
-                       B.row_col.p, B.row_val.p, e->cap_row, e->triv.p, B.dval.p, B.flags.p,
+                       B.row_col.p, B.row_val.p, e->cap_row, e->triv.p, B.dval.p, B.hlist.p,
+                       (int)(e->force_general == 2), B.flags.p, B.counters.p);
+    MA_HIP_TRY(hipMemcpyAsync(&nhuge, B.hlist.p + n, sizeof(int32_t), hipMemcpyDeviceToHost,
+                              e->stream));
+    MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  }
+  if (nhuge > 0) {
+    // the few cells beyond the general kernel: one workgroup each
+    if (mode == 0) {
+      MA_HIP_TRY(B.hrow_col.ensure((size_t)nhuge * kHugeRow));
+      MA_HIP_TRY(B.hrow_val.ensure((size_t)nhuge * kHugeRow));
+    }
+    auto kern = mode == 0 ? k_cell_huge<0> : mode == 1 ? k_cell_huge<1> : k_cell_huge<2>;
+    hipLaunchKernelGGL(kern, dim3(nhuge), dim3(kHugeBlock), 0, e->stream, B.hlist.p, g, n,
+                       e->pos.p, B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p, B.cand.p,
+                       e->tbin_start.p, e->tbin_items.p, e->tri.p, B.mass_s.p, B.intg_s.p,
+                       B.fval_part.p, B.mom_s.p, B.row_cnt.p, B.row_col.p, B.row_val.p,
+                       e->cap_row, B.hrow_col.p, B.hrow_val.p, e->triv.p, B.dval.p, B.flags.p,
                        B.counters.p);
   }
   MA_HIP_TRY(hipEventRecord(e->ev[6], e->stream));
@@ -1051,8 +1630,8 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
     MA_HIP_TRY(B.csr_col.ensure((size_t)nnz + 1));
     MA_HIP_TRY(B.csr_val.ensure((size_t)nnz + 1));
     hipLaunchKernelGGL(k_csr_fill, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, n, B.row_cnt.p,
-                       B.row_col.p, B.row_val.p, e->cap_row, B.csr_ptr.p, B.csr_col.p,
-                       B.csr_val.p, B.diag.p);
+                       B.row_col.p, B.row_val.p, e->cap_row, B.hrow_col.p, B.hrow_val.p,
+                       B.csr_ptr.p, B.csr_col.p, B.csr_val.p, B.diag.p);
     MA_HIP_TRY(device_sum(e, B.fval_part.p, n, B.red_tmp.p + 1024));
     e->nnz = nnz;
   }
@@ -1087,10 +1666,16 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   e->info.nbincand = (int64_t)ct[6];
   e->info.npairs = npairs;
   e->info.ngeneral = ngen;
+  e->info.nhuge = nhuge;
   e->info.nnz = e->nnz;
   if (fl[0]) {
-    e->err = "cell polygon exceeded " + std::to_string(kMaxPoly) + " vertices or " +
-             std::to_string(kMaxRow - 1) + " neighbours";
+    e->err = (fl[0] & 2) ? "a cell x triangle fragment exceeded " + std::to_string(kFragCap) +
+                               " vertices"
+                         : "a cell polygon exceeded " + std::to_string(kHugePoly) +
+                               " vertices (or " + std::to_string(kHugeMaxClips) +
+                               " clips): Morton cell " + std::to_string(ct[7] >> 32) + ", " +
+                               std::to_string((ct[7] >> 16) & 0xffff) + " vertices after " +
+                               std::to_string(ct[7] & 0xffff) + " clips";
     return MA_CAPACITY;
   }
   e->have_eval = true;

@@ -25,12 +25,15 @@ def _pattern(H, rel=0.0):
     return set(zip(*np.nonzero(np.abs(A) > thr)))
 
 
-@pytest.mark.parametrize("path", ["fast", "general"])
+PATHS = {"fast": 0, "general": 1, "huge": 2}
+
+
+@pytest.mark.parametrize("path", list(PATHS))
 @pytest.mark.parametrize("name", GOLDEN_CASES)
 def test_golden_exact(engine, name, path):
     c = load_golden(name)
     print(f"case {name} path {path}", flush=True)
-    engine.set_option("force_general", path == "general")
+    engine.set_option("force_general", PATHS[path])
     engine.set_mesh(c["mesh"]).set_diracs(c["X"])
     try:
         fval, g, H = engine.kantorovich(c["w"])
@@ -64,12 +67,12 @@ def random_problem(seed):
     return mesh, X, w
 
 
-@pytest.mark.parametrize("path", ["fast", "general"])
+@pytest.mark.parametrize("path", list(PATHS))
 @pytest.mark.parametrize("seed", range(12))
 def test_random_vs_oracle(engine, seed, path):
     mesh, X, w = random_problem(seed)
     print(f"seed {seed} N={X.shape[0]} T={mesh.tri.shape[0]} path {path}", flush=True)
-    engine.set_option("force_general", path == "general")
+    engine.set_option("force_general", PATHS[path])
     engine.set_mesh(mesh).set_diracs(X)
     try:
         fval, g, H = engine.kantorovich(w)
@@ -82,6 +85,38 @@ def test_random_vs_oracle(engine, seed, path):
         assert _rel(H.toarray(), oH.toarray()) <= TOL
         # adjacency identical except for entries at the level of ties
         assert _pattern(H, 1e-9) == _pattern(oH, 1e-9)
+
+
+@pytest.mark.parametrize("nring", [59, 200, 450])
+def test_large_cell_vs_oracle(engine, nring):
+    """A Dirac ringed by nring others: its cell is an nring-gon, beyond the
+    register/LDS path (and, for nring > 60, the general kernel), with a
+    Hessian row of nring + 1 entries."""
+    t = 2 * np.pi * (np.arange(nring) + 0.25) / nring
+    ring = np.stack([0.5 + 0.3 * np.cos(t), 0.5 + 0.3 * np.sin(t)], 1)
+    rng = np.random.default_rng(nring)
+    outer = rng.uniform(0.0, 1.0, size=(300, 2))
+    outer = outer[np.hypot(outer[:, 0] - 0.5, outer[:, 1] - 0.5) > 0.36]
+    X = np.concatenate([[[0.5, 0.5]], ring, outer])
+    vxy, tri = configs.grid_mesh(40)
+    mesh = configs.Mesh(vxy, 1.0 + vxy[:, 0] + 0.5 * vxy[:, 1], tri)
+    w = rng.uniform(-1, 1, X.shape[0]) * 1e-4
+    w[: nring + 1] = 0.0  # a regular nring-gon: every ring Dirac is a neighbour
+    print(f"nring {nring} N={X.shape[0]}", flush=True)
+    engine.set_mesh(mesh).set_diracs(X)
+    fval, g, H = engine.kantorovich(w)
+    info = engine.eval_info()
+    m, mom = engine.first_moment(w)
+    of, og, oH = oracle.kantorovich(mesh, X, w)
+    om, omom = oracle.first_moment(mesh, X, w)
+    assert info["max_poly"] == nring
+    assert (info["nhuge"] > 0) == (nring > 60)
+    assert H[0].nnz == nring + 1
+    assert _rel(g, og) <= TOL
+    assert abs(fval - of) <= TOL * abs(of)
+    assert _rel(H.toarray(), oH.toarray()) <= TOL
+    assert _pattern(H, 1e-9) == _pattern(oH, 1e-9)
+    assert np.abs(mom - omom).max() <= TOL * np.abs(omom).max()
 
 
 @pytest.mark.parametrize("wscale", [0.0, 1.0])
