@@ -31,8 +31,8 @@ namespace ma {
 
 constexpr int kMaxFast = 16;
 constexpr int kCellBlock = 128;
-constexpr int kLocalRings = 3;  // site-centric search radius, in bins
-constexpr int kWalkBudget = 192;  // pyramid nodes a cell may visit before the bin path
+constexpr int kLocalRings = 8;  // site-centric search radius, in bins
+constexpr int kWalkBudget = 512;  // pyramid nodes a cell may visit before the bin path
 constexpr int kNodeDiracs = 8;    // pyramid nodes with at most this many Diracs are taken whole
 constexpr double kBoxBins = 8.0;  // FP32 starting box half-width, in bins
 constexpr int kBoxLabel = -3;     // edge label of that box's cut
@@ -111,11 +111,13 @@ struct CellLDS {
     return (double)r * (1.0 + Tol<R>::pad);
   }
   // Box [X0,X1] x [Y0,Y1] (local coordinates) with the labels of its bottom,
-  // right, top and left edges.  In FP32 the box is also cut to kBoxBins bins
-  // around the Dirac (edges labelled kBoxLabel): every vertex then stays
-  // within a few cells of the Dirac, where FP32 local coordinates are
-  // accurate (a vertex on a box edge at distance ~1 would carry an absolute
-  // error of ~1e-7, i.e. ~1e-4 of a cell).  Returns false if nothing is left.
+  // right, top and left edges, cut to kBoxBins bins around the Dirac (edges
+  // labelled kBoxLabel): every vertex then stays within a few cells of the
+  // Dirac, where FP32 local coordinates are accurate (a vertex on a box edge
+  // at distance ~1 would carry an absolute error of ~1e-7, i.e. ~1e-4 of a
+  // cell), and the search radius starts small.  A finished cell that still
+  // touches the cut is handed to the general kernels.  Returns false if
+  // nothing is left.
   __device__ bool init(double X0, double Y0, double X1, double Y1, double half) {
     int lb = -1, lr = -1, lt = -1, ll = -1;
     if (X0 < -half) { X0 = -half; ll = kBoxLabel; }
@@ -142,10 +144,12 @@ struct CellLDS {
   bool cut = false;
 };
 
-// Half-width of the FP32 starting box, in bins (FP64: unbounded).
+// Half-width of the starting box, in bins.  In FP32 it bounds the local
+// coordinates; in both precisions it keeps the polygon radius, and with it
+// the pruning radius of the search, small from the start.
 template <class R>
 __device__ inline double box_cut(const Grid &g) {
-  return sizeof(R) == sizeof(float) ? kBoxBins * g.h : kInf;
+  return kBoxBins * g.h;
 }
 
 // Half-plane of competitor j seen from Dirac k, in coordinates local to y_k:
@@ -186,8 +190,22 @@ __device__ inline int clip_lds(CellLDS<R> &L, int n, const HalfPlane<R> &H, bool
   const uint32_t full = (1u << n) - 1u;
   if (in == full) return n;
   if (in == 0u) return 0;
-  const uint32_t nxt = ((in >> 1) | ((in & 1u) << (n - 1))) & full;
-  const uint32_t exit_m = in & ~nxt, entry_m = ~in & nxt & full;
+  uint32_t nxt = ((in >> 1) | ((in & 1u) << (n - 1))) & full;
+  uint32_t exit_m = in & ~nxt, entry_m = ~in & nxt & full;
+  if (sizeof(R) == sizeof(float) && (__popc(exit_m) != 1 || __popc(entry_m) != 1)) {
+    // FP32 rounding broke the single run: classify every stored vertex in FP64
+    in = 0u;
+    for (int v = 0; v < n; v++) {
+      const double x = (double)L.sx[v][t], y = (double)L.sy[v][t];
+      if (H.kk64 - 2.0 * (x * H.Yx64 + y * H.Yy64) > 0.0) in |= 1u << v;
+    }
+    nredo += n;
+    if (in == full) return n;
+    if (in == 0u) return 0;
+    nxt = ((in >> 1) | ((in & 1u) << (n - 1))) & full;
+    exit_m = in & ~nxt;
+    entry_m = ~in & nxt & full;
+  }
   if (__popc(exit_m) != 1 || __popc(entry_m) != 1) {
     fail = true;
     return n;
@@ -328,6 +346,36 @@ __device__ inline bool far_enough(double D, double rho, double W, double wk) {
   return D > rho && D * D - 2.0 * rho * D - (W - wk) > 1e-12 * (D * D + fabs(W) + fabs(wk));
 }
 
+// Multi-level certificate for the polygon (radius rho about y_k) once the
+// bins within Chebyshev distance r of the Dirac's bin have been visited.  At
+// level l the 3 x 3 block of level-l nodes around the Dirac's node contains
+// the bins within 2^l of its bin; the Diracs of that block not yet covered
+// lie at least as far as the boundary of the covered region and weigh at
+// most the block's maximum (pyr3).  Blocks are nested and the top one is the
+// whole grid, so every unvisited Dirac is bounded once.
+__device__ inline bool certify_levels(const Grid &g, const double *pyr3, const PyrOff &off, int sbx,
+                                      int sby, int r, double yx, double yy, double rho,
+                                      double wk) {
+  int x0 = max(sbx - r, 0), x1 = min(sbx + r, g.G - 1);
+  int y0 = max(sby - r, 0), y1 = min(sby + r, g.G - 1);
+  for (int l = 0; l <= g.L; l++) {
+    const int nx = sbx >> l, ny = sby >> l;
+    const int bx0 = max((nx - 1) << l, 0), bx1 = min(((nx + 2) << l) - 1, g.G - 1);
+    const int by0 = max((ny - 1) << l, 0), by1 = min(((ny + 2) << l) - 1, g.G - 1);
+    if (bx0 > x0 || bx1 < x1 || by0 > y0 || by1 < y1) continue;  // does not hold the region yet
+    double D = kInf;
+    if (x0 > 0) D = fmin(D, yx - (g.x0 + x0 * g.h));
+    if (x1 < g.G - 1) D = fmin(D, g.x0 + (x1 + 1) * g.h - yx);
+    if (y0 > 0) D = fmin(D, yy - (g.y0 + y0 * g.h));
+    if (y1 < g.G - 1) D = fmin(D, g.y0 + (y1 + 1) * g.h - yy);
+    if (D == kInf) return true;  // the covered region is the whole grid
+    const double Wb = pyr3[off.o[l] + morton2((uint32_t)nx, (uint32_t)ny)];
+    if (Wb != -kInf && !far_enough(D, rho, Wb, wk)) return false;
+    x0 = bx0; x1 = bx1; y0 = by0; y1 = by1;
+  }
+  return true;
+}
+
 // Counts the lanes of the wave with flag set: one atomic per wave, issued by
 // the lowest such lane.
 template <class T>
@@ -350,7 +398,8 @@ __device__ inline void wave_count(T *dst, bool flag) {
 // cell.
 template <class R>
 __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const double *pyr,
-                                                           PyrOff off, int rmax) {
+                                                           const double *pyr3, PyrOff off,
+                                                           int rmax) {
   __shared__ R sx[kMaxFast][kCellBlock];
   __shared__ R sy[kMaxFast][kCellBlock];
   __shared__ int sl[kMaxFast][kCellBlock];
@@ -390,7 +439,8 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
     if (sbx + r < g.G - 1) D = fmin(D, g.x0 + (sbx + r + 1) * g.h - yx);
     if (sby - r > 0) D = fmin(D, yy - (g.y0 + (sby - r) * g.h));
     if (sby + r < g.G - 1) D = fmin(D, g.y0 + (sby + r + 1) * g.h - yy);
-    if (D == kInf || far_enough(D, sqrt(rho2), W, wk)) {
+    if (D == kInf || far_enough(D, sqrt(rho2), W, wk) ||
+        certify_levels(g, pyr3, off, sbx, sby, r, yx, yy, sqrt(rho2), wk)) {
       certified = true;
       break;
     }
@@ -448,8 +498,15 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
       m++;
     }
   }
-  // a cell still touching the FP32 starting box is larger than that box
-  if (!fail && !over_budget && L.touches_cut(n)) fail = true;
+  // a cell still touching the starting box is larger than that box (or lies
+  // beyond it): in FP64 it is rebuilt from the per-bin candidate owners, in
+  // FP32 by the FP64 kernels
+  if (!fail && !over_budget && L.touches_cut(n)) {
+    if (sizeof(R) == sizeof(double))
+      over_budget = true;
+    else
+      fail = true;
+  }
   wave_count(&a.flags[2], walk);  // cells that needed the pyramid walk
   wave_count(&a.counters[6], over_budget && !fail);  // cells left to the bin path
   if (Tol<R>::redo > R(0) && nredo > 0) atomicAdd(&a.counters[5], (unsigned long long)nredo);
@@ -500,7 +557,8 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_bins(CellArgs a, const int3
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1], wk = a.w[k];
   int n = 4, nredo = 0;
   bool fail = !L.init(g.x0 + rx0 * g.h - yx, g.y0 + ry0 * g.h - yy,
-                      g.x0 + (rx1 + 1) * g.h - yx, g.y0 + (ry1 + 1) * g.h - yy, box_cut<R>(g));
+                      g.x0 + (rx1 + 1) * g.h - yx, g.y0 + (ry1 + 1) * g.h - yy,
+                      sizeof(R) == sizeof(float) ? box_cut<R>(g) : kInf);  // FP64: the rectangle bounds it
   double rho2 = L.rho2(4);
   const int sbx = min(max(bin_coord(yx, g.x0, g.h, g.G), rx0), rx1);
   const int sby = min(max(bin_coord(yy, g.y0, g.h, g.G), ry0), ry1);
@@ -1032,10 +1090,10 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   for (int l = 0; l <= e->grid.L; l++) po.o[l] = e->pyr_off[l];
   if (fp32)
     hipLaunchKernelGGL(k_cell_local<float>, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0,
-                       e->stream, a, B.pyr.p, po, kLocalRings);
+                       e->stream, a, B.pyr.p, B.pyr3.p, po, kLocalRings);
   else
     hipLaunchKernelGGL(k_cell_local<double>, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0,
-                       e->stream, a, B.pyr.p, po, kLocalRings);
+                       e->stream, a, B.pyr.p, B.pyr3.p, po, kLocalRings);
   {
     int32_t fl0[4];
     unsigned long long ct0[8];

@@ -77,10 +77,15 @@ constexpr int kHugeRow = kHugePoly + 1;
 // per-cell kernels (their candidate loops are sequential) for the
 // one-workgroup-per-cell kernel.
 constexpr int kBigRect = 64;
+// Up to this many cells left by the fast path go straight to the
+// one-workgroup-per-cell kernel.
+constexpr int kHugeRoute = 32768;
+constexpr int kLongRows = 1024;  // pool of staged rows longer than kMaxRow
 
 struct EvalBuffers {
   DBuf<double> w_s;          // weights in Morton order
   DBuf<double> pyr;          // max-weight pyramid (all levels)
+  DBuf<double> pyr3;         // per level: max over the 3 x 3 neighbouring nodes
   DBuf<int32_t> cand_cnt;    // CSR offsets of the per-bin candidate lists (nbins + 1)
   DBuf<int32_t> cand;        // candidate owners, bin after bin
   DBuf<double> ubin;         // per bin: min over Diracs of the max power over the bin
@@ -96,7 +101,9 @@ struct EvalBuffers {
   DBuf<double> csr_val;
   DBuf<double> diag;
   DBuf<int32_t> flags;       // [0] overflow poly, [1] overflow cand, [2] overflow row
-  DBuf<unsigned long long> counters;  // [0] fragments, [1] candidates, [2] max poly, [3] empty
+  DBuf<unsigned long long> counters;  // [0] fragments, [1] candidates, [2] max poly, [3] empty,
+                                      // [4] vertices, [5] FP64 re-decisions, [6] bin-path cells,
+                                      // [7] failing cell (k_cell_huge), [8] long rows
   DBuf<int32_t> scan_tmp;
   DBuf<double> red_tmp;
   // register/LDS fast path

@@ -830,8 +830,14 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
     auto t0 = std::chrono::steady_clock::now();
     double rel = o.cg_tol;
     if (rel <= 0) {
-      // inexact Newton forcing term, tightened as the residual shrinks
-      rel = std::min(1e-3, std::max(1e-12, 1e-2 * ns.l2 / std::max(l2_0, 1e-300)));
+      // inexact Newton forcing term: tightened as the residual shrinks, but
+      // no tighter than the stopping test needs (a tenth of the remaining
+      // reduction; the quadratic term takes care of the rest)
+      const double need = o.stop_rule == MA_STOP_L2 ? o.eps_g / std::max(ns.l2, 1e-300)
+                                                    : o.tol / std::max(ns.maxrel, 1e-300);
+      rel = std::min(1e-2 * ns.l2 / std::max(l2_0, 1e-300), 1e-3);
+      rel = std::max(rel, std::min(0.1 * need, 1e-3));
+      rel = std::max(rel, 1e-12);
     }
     int64_t it = 0;
     double rr = 0.0;
@@ -874,8 +880,13 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
     // close to its accuracy floor (a damped step, or less than a halving of
     // the error, once below kStall)
     if (fp32_phase && (ns.maxrel <= kSwitch ||
-                       (ns0.maxrel < kStall && (alpha < 1.0 || ns.maxrel > 0.5 * ns0.maxrel))))
+                       (ns0.maxrel < kStall && (alpha < 1.0 || ns.maxrel > 0.5 * ns0.maxrel)))) {
+      // near the FP32 accuracy floor the FP32 residual is mostly rounding:
+      // take the next direction from an FP64 evaluation at the same point
       fp32_phase = false;
+      s = eval_at(S.w.p, fval, ns);
+      if (s != MA_OK) return s;
+    }
     if (o.verbose)
       fprintf(stderr, "it %lld: f=%.17g |df|=%.6g maxrel=%.3g tau=%g eval=%lld cg=%lld\n",
               (long long)niter, fval - ns.nuw, ns.l2, ns.maxrel, alpha, (long long)st.neval,

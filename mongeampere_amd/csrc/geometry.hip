@@ -294,6 +294,21 @@ __global__ void k_pyr_leaf(const int32_t *bin_start, const double *w, int64_t nb
   for (int32_t q = bin_start[b]; q < bin_start[b + 1]; q++) m = fmax(m, w[q]);
   lvl0[b] = m;
 }
+// Neighbourhood maxima: out[m] = max of lvl over the 3 x 3 nodes around node
+// m of a level with `side` nodes per side.
+__global__ void k_pyr3(const double *lvl, int side, int64_t nn, double *out) {
+  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (m >= nn) return;
+  const int x = (int)compact1by1((uint32_t)m), y = (int)compact1by1((uint32_t)(m >> 1));
+  double v = -kInf;
+  for (int dy = -1; dy <= 1; dy++)
+    for (int dx = -1; dx <= 1; dx++) {
+      const int xx = x + dx, yy = y + dy;
+      if (xx >= 0 && xx < side && yy >= 0 && yy < side)
+        v = fmax(v, lvl[morton2((uint32_t)xx, (uint32_t)yy)]);
+    }
+  out[m] = v;
+}
 __global__ void k_pyr_up(const double *lo, double *hi, int64_t nhi) {
   int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (m >= nhi) return;
@@ -1049,8 +1064,7 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
   __shared__ double dtmp[kHugeBlock / 64];
   __shared__ int s_empty, s_best;
   const int tid = threadIdx.x;
-  const int slot = blockIdx.x;
-  const int64_t k = hlist[slot];
+  const int64_t k = hlist[blockIdx.x];
   const int rx0 = rect[4 * k], ry0 = rect[4 * k + 1], rx1 = rect[4 * k + 2], ry1 = rect[4 * k + 3];
   const double yx = pos[2 * k], yy = pos[2 * k + 1], wk = w[k];
   int np = 0, cur = 0;
@@ -1104,13 +1118,17 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
           c1 = cand_start[b + 1];
         }
       }
+      // each (bin, candidate) entry clips at most once per batch: near-parallel
+      // lines could otherwise trade rounding-level slivers forever
+      uint64_t used = 0ull;
       while (true) {
         if (tid == 0) s_best = INT_MAX;
         double best = kInf;
-        int32_t bj = -1;
+        int32_t bj = -1, bi = -1;
         for (int32_t i = c0; i < c1; i++) {
           const int32_t j = cand[i];
           if (j == k) continue;
+          if (i - c0 < 64 && ((used >> (i - c0)) & 1ull)) continue;
           const double Yx = pos[2 * j] - yx, Yy = pos[2 * j + 1] - yy, wj = w[j];
           const double d2 = Yx * Yx + Yy * Yy;
           if (d2 == 0.0) {
@@ -1130,6 +1148,7 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
           if (depth < best) {
             best = depth;
             bj = j;
+            bi = i - c0;
           }
         }
         // deepest over the block; the smallest Dirac index among equals
@@ -1142,6 +1161,7 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
           break;
         }
         if (j == INT_MAX) break;
+        if (bj == j && bi < 64) used |= 1ull << bi;
         if (++nclip > kHugeMaxClips) {
           fail = true;
           break;
@@ -1399,10 +1419,18 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
     int32_t *oc = row_col + k * (int64_t)cap_row;
     double *ov = row_val + k * (int64_t)cap_row;
     if (cnt > cap_row) {
-      // long row: staged at its slot; the first regular entry points there
-      if (tid == 0) oc[0] = slot;
-      oc = hrow_col + (int64_t)slot * kHugeRow;
-      ov = hrow_val + (int64_t)slot * kHugeRow;
+      // long row: staged in a slot of the long-row pool; the first regular
+      // entry points there
+      if (tid == 0) s_best = (int)atomicAdd(&counters[8], 1ull);
+      __syncthreads();
+      const int lslot = s_best;
+      if (lslot >= kLongRows) {
+        if (tid == 0) atomicOr(&flags[0], 4);
+        return;
+      }
+      if (tid == 0) oc[0] = lslot;
+      oc = hrow_col + (int64_t)lslot * kHugeRow;
+      ov = hrow_val + (int64_t)lslot * kHugeRow;
     }
     for (int c = 0; c < kHugeChunk; c++) {
       if (!keep[c]) continue;
@@ -1552,6 +1580,7 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   PyrOff po{};
   for (int l = 0; l <= g.L; l++) po.o[l] = e->pyr_off[l];
   MA_HIP_TRY(B.pyr.ensure((size_t)tot));
+  MA_HIP_TRY(B.pyr3.ensure((size_t)tot));
   MA_HIP_TRY(B.cand_cnt.ensure((size_t)g.nbins + 1));
   MA_HIP_TRY(B.owner.ensure((size_t)g.nbins));
   MA_HIP_TRY(B.rect.ensure((size_t)(4 * n)));
@@ -1563,7 +1592,7 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   MA_HIP_TRY(B.row_col.ensure((size_t)n * e->cap_row));
   MA_HIP_TRY(B.row_val.ensure((size_t)n * e->cap_row));
   MA_HIP_TRY(B.flags.ensure(4));
-  MA_HIP_TRY(B.counters.ensure(8));
+  MA_HIP_TRY(B.counters.ensure(12));
   MA_HIP_TRY(B.diag.ensure((size_t)n));
   MA_HIP_TRY(B.red_tmp.ensure(2048));
   if (!e->have_eval) {
@@ -1578,17 +1607,29 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
     hipLaunchKernelGGL(k_pyr_up, dim3(nblocks(nh)), dim3(kBlock), 0, e->stream,
                        B.pyr.p + e->pyr_off[l - 1], B.pyr.p + e->pyr_off[l], nh);
   }
+  for (int l = 0; l <= g.L; l++) {
+    const int64_t nn = (int64_t)1 << (2 * (g.L - l));
+    hipLaunchKernelGGL(k_pyr3, dim3(nblocks(nn)), dim3(kBlock), 0, e->stream,
+                       B.pyr.p + e->pyr_off[l], g.G >> l, nn, B.pyr3.p + e->pyr_off[l]);
+  }
   MA_HIP_TRY(hipEventRecord(e->ev[1], e->stream));
   MA_HIP_TRY(hipMemsetAsync(B.flags.p, 0, sizeof(int32_t) * 4, e->stream));
-  MA_HIP_TRY(hipMemsetAsync(B.counters.p, 0, sizeof(unsigned long long) * 8, e->stream));
+  MA_HIP_TRY(hipMemsetAsync(B.counters.p, 0, sizeof(unsigned long long) * 12, e->stream));
   MA_HIP_TRY(hipEventRecord(e->ev[2], e->stream));
   int32_t npairs = 0, ngen = 0;
   ma_status cs = run_cells(e, mode, fp32, &npairs, &ngen);
   if (cs != MA_OK) return cs;
   int32_t nhuge = 0;
-  if (ngen > 0) {
+  const int32_t *hl = nullptr;  // list of the cells for k_cell_huge
+  if (ngen > 0 && ngen <= kHugeRoute && e->force_general != 1) {
+    // few cells: one workgroup each beats one thread each (whose candidate
+    // loops run at memory latency)
+    nhuge = ngen;
+    hl = B.glist.p;
+  } else if (ngen > 0) {
     // cells that overflowed the fast path
     MA_HIP_TRY(B.hlist.ensure((size_t)n + 1));
+    hl = B.hlist.p;
     MA_HIP_TRY(hipMemsetAsync(B.hlist.p + n, 0, sizeof(int32_t), e->stream));
     auto kern = mode == 0 ? k_cells_general<0> : mode == 1 ? k_cells_general<1>
                                                             : k_cells_general<2>;
@@ -1605,11 +1646,11 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   if (nhuge > 0) {
     // the few cells beyond the general kernel: one workgroup each
     if (mode == 0) {
-      MA_HIP_TRY(B.hrow_col.ensure((size_t)nhuge * kHugeRow));
-      MA_HIP_TRY(B.hrow_val.ensure((size_t)nhuge * kHugeRow));
+      MA_HIP_TRY(B.hrow_col.ensure((size_t)kLongRows * kHugeRow));
+      MA_HIP_TRY(B.hrow_val.ensure((size_t)kLongRows * kHugeRow));
     }
     auto kern = mode == 0 ? k_cell_huge<0> : mode == 1 ? k_cell_huge<1> : k_cell_huge<2>;
-    hipLaunchKernelGGL(kern, dim3(nhuge), dim3(kHugeBlock), 0, e->stream, B.hlist.p, g, n,
+    hipLaunchKernelGGL(kern, dim3(nhuge), dim3(kHugeBlock), 0, e->stream, hl, g, n,
                        e->pos.p, B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p, B.cand.p,
                        e->tbin_start.p, e->tbin_items.p, e->tri.p, B.mass_s.p, B.intg_s.p,
                        B.fval_part.p, B.mom_s.p, B.row_cnt.p, B.row_col.p, B.row_val.p,
@@ -1669,7 +1710,9 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   e->info.nhuge = nhuge;
   e->info.nnz = e->nnz;
   if (fl[0]) {
-    e->err = (fl[0] & 2) ? "a cell x triangle fragment exceeded " + std::to_string(kFragCap) +
+    e->err = (fl[0] & 4) ? "more than " + std::to_string(kLongRows) +
+                               " Hessian rows longer than " + std::to_string(kMaxRow) + " entries"
+           : (fl[0] & 2) ? "a cell x triangle fragment exceeded " + std::to_string(kFragCap) +
                                " vertices"
                          : "a cell polygon exceeded " + std::to_string(kHugePoly) +
                                " vertices (or " + std::to_string(kHugeMaxClips) +

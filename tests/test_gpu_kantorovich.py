@@ -87,11 +87,14 @@ def test_random_vs_oracle(engine, seed, path):
         assert _pattern(H, 1e-9) == _pattern(oH, 1e-9)
 
 
+@pytest.mark.parametrize("path", ["default", "general"])
 @pytest.mark.parametrize("nring", [59, 200, 450])
-def test_large_cell_vs_oracle(engine, nring):
+def test_large_cell_vs_oracle(engine, nring, path):
     """A Dirac ringed by nring others: its cell is an nring-gon, beyond the
     register/LDS path (and, for nring > 60, the general kernel), with a
-    Hessian row of nring + 1 entries."""
+    Hessian row of nring + 1 entries.  By default the few such cells go to
+    the workgroup-per-cell kernel; forced through the general kernel, only
+    those beyond its capacity do."""
     t = 2 * np.pi * (np.arange(nring) + 0.25) / nring
     ring = np.stack([0.5 + 0.3 * np.cos(t), 0.5 + 0.3 * np.sin(t)], 1)
     rng = np.random.default_rng(nring)
@@ -102,15 +105,22 @@ def test_large_cell_vs_oracle(engine, nring):
     mesh = configs.Mesh(vxy, 1.0 + vxy[:, 0] + 0.5 * vxy[:, 1], tri)
     w = rng.uniform(-1, 1, X.shape[0]) * 1e-4
     w[: nring + 1] = 0.0  # a regular nring-gon: every ring Dirac is a neighbour
-    print(f"nring {nring} N={X.shape[0]}", flush=True)
+    print(f"nring {nring} N={X.shape[0]} path {path}", flush=True)
     engine.set_mesh(mesh).set_diracs(X)
-    fval, g, H = engine.kantorovich(w)
-    info = engine.eval_info()
-    m, mom = engine.first_moment(w)
+    engine.set_option("force_general", 1 if path == "general" else 0)
+    try:
+        fval, g, H = engine.kantorovich(w)
+        info = engine.eval_info()
+        m, mom = engine.first_moment(w)
+    finally:
+        engine.set_option("force_general", 0)
     of, og, oH = oracle.kantorovich(mesh, X, w)
     om, omom = oracle.first_moment(mesh, X, w)
     assert info["max_poly"] == nring
-    assert (info["nhuge"] > 0) == (nring > 60)
+    if path == "general":
+        assert (info["nhuge"] > 0) == (nring > 60)
+    else:
+        assert info["nhuge"] >= 1
     assert H[0].nnz == nring + 1
     assert _rel(g, og) <= TOL
     assert abs(fval - of) <= TOL * abs(of)
