@@ -193,6 +193,15 @@ def main():
         e.kantorovich_device(w_dev.data_ptr(), other)
         ms_o, _, _, _ = time_evals(e, w_dev.data_ptr(), other, max(3, args.steps // 2))
         out[f"evals_per_s_{other}_w0"] = 1000.0 / ms_o
+        # the same evaluation through the host-buffer entry point: weights in
+        # and masses out over PCIe (not the headline value)
+        w_host = np.zeros(N)
+        e.kantorovich(w_host, prec, hessian=False)
+        t0 = time.perf_counter()
+        nh = max(3, args.steps // 2)
+        for _ in range(nh):
+            e.kantorovich(w_host, prec, hessian=False)
+        out[f"evals_per_s_{prec}_w0_host_buffers"] = nh / (time.perf_counter() - t0)
     if not args.no_solve:
         w_star = None
         for q in [q for q in args.solve_precisions.split(",") if q]:

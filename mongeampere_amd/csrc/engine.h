@@ -46,6 +46,15 @@ struct DBuf {
     p = nullptr;
     n = 0;
   }
+  // exchanges the allocations (pointer and capacity together)
+  void swap(DBuf &o) noexcept {
+    T *tp = p;
+    p = o.p;
+    o.p = tp;
+    size_t tn = n;
+    n = o.n;
+    o.n = tn;
+  }
 };
 
 // Uniform square grid of 2^L x 2^L bins over a square region; bins are indexed
@@ -117,6 +126,7 @@ struct EvalBuffers {
 
 struct SolverBuffers {
   DBuf<double> w, m, nu, g, d, x, r, z, p, Ap, wtrial, red;
+  DBuf<double> stage;         // host <-> device staging of per-Dirac arrays
   DBuf<int32_t> hptr, hcol;   // caller-supplied CSR (ma_solve_laplacian_csr)
   DBuf<double> hval, hg, hd;
 };

@@ -875,7 +875,7 @@ ma_status ma_ot_solve(ma_engine *e, const double *nu, double *w, int init_w,
       }
     }
     if (result != MA_OK) break;
-    std::swap(S.w.p, S.wtrial.p);
+    S.w.swap(S.wtrial);
     // leave the FP32 phase at kSwitch, or earlier when FP32 progress stalls
     // close to its accuracy floor (a damped step, or less than a halving of
     // the error, once below kStall)

@@ -116,10 +116,10 @@ __global__ void k_scatter_f64(const double *src, const int32_t *perm, double *ds
 hipError_t to_morton(ma_engine *e, const double *src, double *dst, bool src_on_device) {
   const double *s = src;
   if (!src_on_device) {
-    MA_HIP_RET(e->sb.wtrial.ensure((size_t)e->n));
-    MA_HIP_RET(hipMemcpyAsync(e->sb.wtrial.p, src, sizeof(double) * e->n, hipMemcpyHostToDevice,
+    MA_HIP_RET(e->sb.stage.ensure((size_t)e->n));
+    MA_HIP_RET(hipMemcpyAsync(e->sb.stage.p, src, sizeof(double) * e->n, hipMemcpyHostToDevice,
                               e->stream));
-    s = e->sb.wtrial.p;
+    s = e->sb.stage.p;
   }
   hipLaunchKernelGGL(k_gather_f64, dim3(nblocks(e->n)), dim3(kBlock), 0, e->stream, s,
                      e->perm.p, dst, e->n);
@@ -132,10 +132,10 @@ hipError_t from_morton(ma_engine *e, const double *src, double *dst, bool dst_on
                        e->perm.p, dst, e->n);
     return hipGetLastError();
   }
-  MA_HIP_RET(e->sb.wtrial.ensure((size_t)e->n));
+  MA_HIP_RET(e->sb.stage.ensure((size_t)e->n));
   hipLaunchKernelGGL(k_scatter_f64, dim3(nblocks(e->n)), dim3(kBlock), 0, e->stream, src,
-                     e->perm.p, e->sb.wtrial.p, e->n);
-  MA_HIP_RET(hipMemcpyAsync(dst, e->sb.wtrial.p, sizeof(double) * e->n, hipMemcpyDeviceToHost,
+                     e->perm.p, e->sb.stage.p, e->n);
+  MA_HIP_RET(hipMemcpyAsync(dst, e->sb.stage.p, sizeof(double) * e->n, hipMemcpyDeviceToHost,
                             e->stream));
   return hipStreamSynchronize(e->stream);
 }

@@ -98,3 +98,24 @@ def test_empty_cell_status(engine):
     w, st = engine.ot_solve(p.nu, raise_on_error=False)
     assert st.status == "MA_EMPTY_CELL"
     assert st.bad_index in (3, 7)
+
+
+def test_engine_reuse_across_sizes():
+    """One engine solving problems of growing and shrinking size (both
+    precisions) gives the same weights as a fresh engine each time: buffers
+    exchanged by the Newton driver keep their capacities."""
+    from mongeampere_amd.api import Engine
+    probs = [configs.config1(), configs.config3(N=20_000, n=128), configs.config1(N=400, n=24),
+             configs.config3(N=30_000, n=160)]
+    shared = Engine(0)
+    for i, p in enumerate(probs):
+        for prec in ("mixed", "fp64"):
+            print(f"problem {i} N={p.X.shape[0]} {prec}", flush=True)
+            shared.set_mesh(p.mesh).set_diracs(p.X)
+            w1, s1 = shared.ot_solve(p.nu, stop="maxrel", tol=1e-9, precision=prec)
+            with Engine(0) as fresh:
+                fresh.set_mesh(p.mesh).set_diracs(p.X)
+                w2, s2 = fresh.ot_solve(p.nu, stop="maxrel", tol=1e-9, precision=prec)
+            assert s1.status == s2.status == "MA_OK"
+            assert np.max(np.abs(w1 - w2)) <= 1e-9 * np.max(np.abs(w2)) + 1e-15
+    shared.close()
