@@ -127,6 +127,7 @@ struct EvalBuffers {
 struct SolverBuffers {
   DBuf<double> w, m, nu, g, d, x, r, z, p, Ap, wtrial, red;
   DBuf<double> stage;         // host <-> device staging of per-Dirac arrays
+  DBuf<double> red_sp;        // partial sums of the sparse products
   DBuf<int32_t> hptr, hcol;   // caller-supplied CSR (ma_solve_laplacian_csr)
   DBuf<double> hval, hg, hd;
 };

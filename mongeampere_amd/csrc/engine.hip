@@ -16,6 +16,7 @@ hipError_t device_sum(ma_engine *e, const double *x, int64_t n, double *d_out);
 
 constexpr int kVB = 256;      // threads per block for vector kernels
 constexpr int kVGrid = 512;   // fixed grid => fixed number of partial sums
+constexpr int kSpGrid = 4096; // blocks of the sparse products (gathers need more waves in flight)
 
 // ---------------------------------------------------------------------------
 // Mesh preparation: ccw orientation and per-face linear density a x + b y + c
@@ -96,7 +97,7 @@ __global__ void __launch_bounds__(kVB) k_spmv_dot(int64_t n, const int32_t *ptr,
   if (scal[5] != 0.0) return;
   double acc = 0.0;
   const int lane = (int)(threadIdx.x % kRowLanes);
-  const int64_t stride = (int64_t)kVGrid * kVB / kRowLanes;
+  const int64_t stride = (int64_t)kSpGrid * kVB / kRowLanes;
   // the kRowLanes lanes of a group share i, so they iterate together
   for (int64_t i = (blockIdx.x * (int64_t)kVB + threadIdx.x) / kRowLanes; i < n; i += stride) {
     const double v = row_dot(ptr, col, val, x, i, true, lane);
@@ -107,6 +108,16 @@ __global__ void __launch_bounds__(kVB) k_spmv_dot(int64_t n, const int32_t *ptr,
   }
   acc = block_sum(acc, s);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+// Folds the kSpGrid partial sums of k_spmv_dot into kVGrid partials (in a
+// fixed order, so that sums stay deterministic).
+__global__ void k_fold_partials(const double *in, double *out, const double *scal) {
+  if (scal[5] != 0.0) return;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= kVGrid) return;
+  double v = 0.0;
+  for (int i = 0; i < kSpGrid / kVGrid; i++) v += in[b * (kSpGrid / kVGrid) + i];
+  out[b] = v;
 }
 // x = 0, r = b (r[gi] = 0), partial sums of r.r; clears the done flag
 __global__ void __launch_bounds__(kVB) k_cg_begin(int64_t n, const double *b, int32_t gi,
@@ -311,6 +322,8 @@ static hipError_t pcg(ma_engine *e, const CsrView &M, int32_t gi, const double *
   MA_HIP_RET(S.Ap.ensure((size_t)n));
   MA_HIP_RET(S.red.ensure(4 * kVGrid + 16));
   double *part_a = S.red.p, *part_b = S.red.p + kVGrid, *part_c = S.red.p + 2 * kVGrid;
+  MA_HIP_RET(S.red_sp.ensure(kSpGrid));
+  double *part_sp = S.red_sp.p;
   double *scal = S.red.p + 4 * kVGrid;
   if (!e->amg) e->amg = amg_create();
   {
@@ -346,8 +359,10 @@ static hipError_t pcg(ma_engine *e, const CsrView &M, int32_t gi, const double *
   auto launch = [&](int it0, int count) -> hipError_t {
     for (int q = 0; q < count; q++) {
       const int parity = (it0 + q) & 1;
-      hipLaunchKernelGGL(k_spmv_dot, dim3(kVGrid), dim3(kVB), 0, e->stream, n, ptr, col, val,
-                         S.p.p, S.Ap.p, part_c, scal);
+      hipLaunchKernelGGL(k_spmv_dot, dim3(kSpGrid), dim3(kVB), 0, e->stream, n, ptr, col, val,
+                         S.p.p, S.Ap.p, part_sp, scal);
+      hipLaunchKernelGGL(k_fold_partials, dim3(kVGrid / kVB), dim3(kVB), 0, e->stream, part_sp,
+                         part_c, scal);
       hipLaunchKernelGGL(k_cg_update, dim3(kVGrid), dim3(kVB), 0, e->stream, n, parity, part_c,
                          scal, x, S.r.p, S.p.p, S.Ap.p, part_b);
       MA_HIP_RET(precondition());
