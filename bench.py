@@ -202,6 +202,12 @@ def main():
         for _ in range(nh):
             e.kantorovich(w_host, prec, hessian=False)
         out[f"evals_per_s_{prec}_w0_host_buffers"] = nh / (time.perf_counter() - t0)
+        # MA::second_moment (FP64; masses, first and second moments to the host)
+        e.second_moment(w_host)
+        t0 = time.perf_counter()
+        for _ in range(nh):
+            e.second_moment(w_host)
+        out["second_moment_per_s_w0_host_buffers"] = nh / (time.perf_counter() - t0)
     if not args.no_solve:
         w_star = None
         for q in [q for q in args.solve_precisions.split(",") if q]:

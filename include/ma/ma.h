@@ -159,6 +159,14 @@ ma_status ma_solve_laplacian_csr(ma_engine *e, int64_t n, const int32_t *rowptr,
  * MA::first_moment (include/MA/lloyd.hpp:30-69).  mom is n x 2 interleaved. */
 ma_status ma_first_moment(ma_engine *e, const double *w, double *mass, double *mom);
 
+/* Cell masses, first moments (int rho x, int rho y; n x 2 interleaved) and
+ * second moments (int rho x^2, int rho y^2, int rho x y; n x 3 interleaved),
+ * replaces MA::second_moment (include/MA/lloyd.hpp:71-123), whose outputs
+ * (masses, centroids, inertia) carry these raw integrals.  Any output may be
+ * NULL. */
+ma_status ma_second_moment(ma_engine *e, const double *w, double *mass, double *mom,
+                           double *inertia);
+
 /* Adjoint of the transport cost.  With the dual value
  *   Phi(Y, w, rho) = int rho(x) min_i (|x - y_i|^2 - w_i) dx,
  * W2^2(rho, sum_i nu_i delta_{y_i}) = max_w Phi + <nu, w>, so at the optimal

@@ -59,6 +59,7 @@ SIGNATURES = [
     ("ma_solve_laplacian", I32, [P, P, P, F64, I32, P, P]),
     ("ma_solve_laplacian_csr", I32, [P, I64, P, P, P, P, P, F64, I32, P, P]),
     ("ma_first_moment", I32, [P, P, P, P]),
+    ("ma_second_moment", I32, [P, P, P, P, P]),
     ("ma_adjoint", I32, [P, P, P, P]),
     ("ma_get_eval_info", I32, [P, P]),
     ("ma_cell_paths", I32, [P, I64, P, P]),

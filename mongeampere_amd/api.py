@@ -8,8 +8,8 @@ Reference entry points and their counterparts here:
   (``include/MA/optimal_transport.hpp:41-87``) -> ``solve_laplacian_matrix(engine, g)``
 * ``MA::ot_solve(T, F, X, masses, x, eps_g, maxiter, verbose, stats)``
   (``include/MA/optimal_transport.hpp:89-193``) -> ``ot_solve(mesh, X, masses, ...)``
-* ``MA::first_moment`` / ``MA::lloyd`` (``include/MA/lloyd.hpp:30-144``)
-                                                -> ``first_moment`` / ``lloyd``
+* ``MA::first_moment`` / ``MA::second_moment`` / ``MA::lloyd``
+  (``include/MA/lloyd.hpp:30-144``)             -> ``first_moment`` / ``second_moment`` / ``lloyd``
 
 Everything runs on the GPU through ``libmagpu.so``; this module only moves
 arrays across the C ABI.
@@ -227,6 +227,17 @@ class Engine:
         self._check(self._L.ma_first_moment(self._h, _ptr(w), _ptr(m), _ptr(mom)))
         return m, mom
 
+    def second_moment(self, w):
+        """``MA::second_moment`` (include/MA/lloyd.hpp:71-123): masses, first
+        moments (n x 2) and second moments (n x 3: int rho x^2, int rho y^2,
+        int rho x y)."""
+        w = np.ascontiguousarray(w, np.float64)
+        m = np.empty(self.n)
+        mom = np.empty((self.n, 2))
+        inertia = np.empty((self.n, 3))
+        self._check(self._L.ma_second_moment(self._h, _ptr(w), _ptr(m), _ptr(mom), _ptr(inertia)))
+        return m, mom, inertia
+
     def lloyd(self, w):
         m, mom = self.first_moment(w)
         with np.errstate(divide="ignore", invalid="ignore"):
@@ -275,3 +286,8 @@ def first_moment(mesh, X, weights):
 
 def lloyd(mesh, X, weights):
     return _engine(mesh, X).lloyd(weights)
+
+
+def second_moment(mesh, X, weights):
+    """``MA::second_moment``: (masses, first moments, second moments)."""
+    return _engine(mesh, X).second_moment(weights)

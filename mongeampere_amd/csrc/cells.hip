@@ -73,6 +73,7 @@ struct CellArgs {
   double *bbox;      // absolute bounding box of the cell (4 per Dirac)
   int32_t *tcount;   // candidate triangles per cell
   double *mass, *intg, *mom;  // mom: local first moments, interleaved (x, y)
+  double *mom2;      // local second moments (xx, yy, xy) per Dirac (mode 3)
   double *eacc;      // edge integrals int rho ds, [v * n + k]
   float *eaccf;      // FP32 edge integrals (mixed mode)
   int32_t *ehit;     // edge seen in a fragment
@@ -337,6 +338,11 @@ __device__ inline void zero_cell_outputs(const CellArgs &a, int64_t k) {
   a.intg[k] = 0.0;
   a.mom[2 * k] = 0.0;
   a.mom[2 * k + 1] = 0.0;
+  if (a.mom2) {
+    a.mom2[3 * k] = 0.0;
+    a.mom2[3 * k + 1] = 0.0;
+    a.mom2[3 * k + 2] = 0.0;
+  }
 }
 
 // Lower bound of pi_j - pi_k over the polygon (radius rho about y_k) for
@@ -701,6 +707,7 @@ struct Integrator {
   int cnt;
   SV<R> first, prev;
   R area, mass, intg, mx, my;
+  R sxx, syy, sxy;    // second moments (MODE 3)
   R cmass, cintg;     // Kahan compensations (FP32)
   // rho(Dirac + X) = r0 + ga (X - tx) + gb (Y - ty): the linear density
   // anchored at the triangle's first vertex (tx, ty), so that its values
@@ -765,6 +772,7 @@ struct Integrator {
     AC6T<R> q;
     ac6_points(x0, y0, x1, y1, x2, y2, q);
     R a1 = R(0), a9 = R(0), b1 = R(0), b9 = R(0);
+    R u1 = R(0), u9 = R(0), v1 = R(0), v9 = R(0), w1 = R(0), w9 = R(0);
     double c1[3] = {0.0, 0.0, 0.0}, c9[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int p = 0; p < 6; p++) {
@@ -789,12 +797,29 @@ struct Integrator {
           if (p < 3) c1[v] += f; else c9[v] += f;
         }
       }
+      if (MODE == 3) {
+        const R fx = rp * q.px[p], fy = rp * q.py[p];
+        if (p < 3) {
+          u1 += fx * q.px[p];
+          v1 += fy * q.py[p];
+          w1 += fx * q.py[p];
+        } else {
+          u9 += fx * q.px[p];
+          v9 += fy * q.py[p];
+          w9 += fx * q.py[p];
+        }
+      }
     }
     if (MODE == 0) {
       acc_add(intg, cintg, ar * (a1 * R(1.0 / 30.0) + a9 * R(9.0 / 30.0)));
     } else {
       mx += ar * (a1 * R(1.0 / 30.0) + a9 * R(9.0 / 30.0));
       my += ar * (b1 * R(1.0 / 30.0) + b9 * R(9.0 / 30.0));
+    }
+    if (MODE == 3) {
+      sxx += ar * (u1 * R(1.0 / 30.0) + u9 * R(9.0 / 30.0));
+      syy += ar * (v1 * R(1.0 / 30.0) + v9 * R(9.0 / 30.0));
+      sxy += ar * (w1 * R(1.0 / 30.0) + w9 * R(9.0 / 30.0));
     }
     if (MODE == 2) {
 #pragma unroll
@@ -848,6 +873,7 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   Integrator<MODE, R> I;
   I.cnt = 0;
   I.area = I.mass = I.intg = I.mx = I.my = R(0);
+  I.sxx = I.syy = I.sxy = R(0);
   I.cmass = I.cintg = R(0);
   I.r0 = (R)(t.a * t.x[0] + t.b * t.y[0] + t.c);  // the value at vertex 0
   I.ga = (R)t.a;
@@ -915,6 +941,11 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
       atomicAdd(&a.mass[k], (double)I.mass);
       atomicAdd(&a.mom[2 * k], (double)I.mx);
       atomicAdd(&a.mom[2 * k + 1], (double)I.my);
+    }
+    if (MODE == 3) {
+      atomicAdd(&a.mom2[3 * k], (double)I.sxx);
+      atomicAdd(&a.mom2[3 * k + 1], (double)I.syy);
+      atomicAdd(&a.mom2[3 * k + 2], (double)I.sxy);
     }
     if (MODE == 2) {
       const int64_t ti = pair_tri[p];
@@ -1005,13 +1036,21 @@ __global__ void k_rows(CellArgs a, double *fval_part, int32_t *row_cnt, int32_t 
   }
 }
 
-// first moments: local (relative to the Dirac) -> absolute
-__global__ void k_mom_finalize(CellArgs a) {
+// moments: local (relative to the Dirac) -> absolute; with `second`, the
+// second moments too (from the local first moments, before those change):
+// int rho x^2 = S_uu + 2 y_x M_u + y_x^2 m, int rho x y = S_uv + y_x M_v + y_y M_u + y_x y_y m
+__global__ void k_mom_finalize(CellArgs a, int second) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= a.n || a.pn[k] < 0) return;
-  const double m = a.mass[k];
-  a.mom[2 * k] += a.pos[2 * k] * m;
-  a.mom[2 * k + 1] += a.pos[2 * k + 1] * m;
+  const double m = a.mass[k], yx = a.pos[2 * k], yy = a.pos[2 * k + 1];
+  const double mu = a.mom[2 * k], mv = a.mom[2 * k + 1];
+  if (second) {
+    a.mom2[3 * k] += 2.0 * yx * mu + yx * yx * m;
+    a.mom2[3 * k + 1] += 2.0 * yy * mv + yy * yy * m;
+    a.mom2[3 * k + 2] += yx * mv + yy * mu + yx * yy * m;
+  }
+  a.mom[2 * k] = mu + yx * m;
+  a.mom[2 * k + 1] = mv + yy * m;
 }
 
 // Lists the cells whose vertex count holds the marker `value`.
@@ -1075,6 +1114,8 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   a.mass = B.mass_s.p;
   a.intg = B.intg_s.p;
   a.mom = B.mom_s.p;
+  if (mode == 3) MA_HIP_TRY(B.mom2_s.ensure((size_t)(3 * n)));
+  a.mom2 = mode == 3 ? B.mom2_s.p : nullptr;
   a.eacc = B.eacc.p;
   a.eaccf = B.eaccf.p;
   a.ehit = B.ehit.p;
@@ -1150,6 +1191,8 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
       hipLaunchKernelGGL((k_frag<0, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
     else if (mode == 1)
       hipLaunchKernelGGL((k_frag<1, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
+    else if (mode == 3)
+      hipLaunchKernelGGL((k_frag<3, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
     else
       hipLaunchKernelGGL((k_frag<2, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
   }
@@ -1160,8 +1203,9 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   else if (mode == 0)
     hipLaunchKernelGGL(k_rows<double>, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a,
                        B.fval_part.p, B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row);
-  else if (mode == 1)  // mode 2 keeps the moments about the Dirac
-    hipLaunchKernelGGL(k_mom_finalize, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a);
+  else if (mode == 1 || mode == 3)  // mode 2 keeps the moments about the Dirac
+    hipLaunchKernelGGL(k_mom_finalize, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a,
+                       (int)(mode == 3));
   int32_t ng = 0;
   if (fl[3]) {
     MA_HIP_TRY(hipMemcpyAsync(B.glist.p + n, &zero, sizeof(int32_t), hipMemcpyHostToDevice,

@@ -102,6 +102,7 @@ struct EvalBuffers {
   DBuf<int32_t> rect;        // per Dirac: bin rectangle (x0, y0, x1, y1)
   DBuf<double> mass_s, intg_s, fval_part;
   DBuf<double> mom_s;        // first moments (2 per Dirac)
+  DBuf<double> mom2_s;       // second moments (3 per Dirac, mode 3)
   DBuf<double> dval;         // adjoint w.r.t. vertex density values (nv)
   DBuf<int32_t> row_cnt;     // per Dirac, entries incl. diagonal
   DBuf<int32_t> row_col;     // n x cap_row

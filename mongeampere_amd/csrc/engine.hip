@@ -270,9 +270,9 @@ __global__ void k_add_const(int64_t n, double *x, double c) {
   if (i < n) x[i] += c;
 }
 
-__global__ void k_split2(const double *m2, int64_t n, int c, double *o) {
+__global__ void k_split(const double *m, int64_t n, int ncol, int c, double *o) {
   int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k < n) o[k] = m2[2 * k + c];
+  if (k < n) o[k] = m[ncol * k + c];
 }
 
 struct NewtonStats {
@@ -654,6 +654,21 @@ ma_status ma_hessian_copy(ma_engine *e, int32_t *rowptr, int32_t *col, double *v
   return MA_OK;
 }
 
+// Copies an interleaved per-Dirac array of `ncol` components from Morton
+// order on the device to the caller's order on the host.
+static hipError_t columns_to_host(ma_engine *e, const double *src, int ncol, double *dst) {
+  std::vector<double> col((size_t)e->n);
+  DBuf<double> tmp;
+  MA_HIP_RET(tmp.ensure((size_t)e->n));
+  for (int c = 0; c < ncol; c++) {
+    hipLaunchKernelGGL(k_split, dim3((unsigned)((e->n + 127) / 128)), dim3(128), 0, e->stream,
+                       src, e->n, ncol, c, tmp.p);
+    MA_HIP_RET(from_morton(e, tmp.p, col.data(), false));
+    for (int64_t i = 0; i < e->n; i++) dst[ncol * i + c] = col[i];
+  }
+  return hipSuccess;
+}
+
 ma_status ma_first_moment(ma_engine *e, const double *w, double *mass, double *mom) {
   ma_status s = need(e);
   if (s != MA_OK) return s;
@@ -663,22 +678,22 @@ ma_status ma_first_moment(ma_engine *e, const double *w, double *mass, double *m
   s = evaluate(e, 1, nullptr);
   if (s != MA_OK) return s;
   if (mass) MA_HIP_TRY(from_morton(e, e->eb.mass_s.p, mass, false));
-  if (mom) {
-    // interleaved (x, y) per Dirac: scatter both components
-    std::vector<double> mx(e->n), my(e->n);
-    DBuf<double> tmp;
-    MA_HIP_TRY(tmp.ensure((size_t)e->n));
-    for (int c = 0; c < 2; c++) {
-      hipLaunchKernelGGL(k_split2, dim3((unsigned)((e->n + 127) / 128)), dim3(128), 0,
-                         e->stream, e->eb.mom_s.p, e->n, c, tmp.p);
-      MA_HIP_TRY(from_morton(e, tmp.p, c == 0 ? mx.data() : my.data(), false));
-    }
-    tmp.release();
-    for (int64_t i = 0; i < e->n; i++) {
-      mom[2 * i] = mx[i];
-      mom[2 * i + 1] = my[i];
-    }
-  }
+  if (mom) MA_HIP_TRY(columns_to_host(e, e->eb.mom_s.p, 2, mom));
+  return MA_OK;
+}
+
+ma_status ma_second_moment(ma_engine *e, const double *w, double *mass, double *mom,
+                           double *inertia) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!w) return MA_BAD_INPUT;
+  MA_HIP_TRY(e->eb.w_s.ensure((size_t)e->n));
+  MA_HIP_TRY(to_morton(e, w, e->eb.w_s.p, false));
+  s = evaluate(e, 3, nullptr);
+  if (s != MA_OK) return s;
+  if (mass) MA_HIP_TRY(from_morton(e, e->eb.mass_s.p, mass, false));
+  if (mom) MA_HIP_TRY(columns_to_host(e, e->eb.mom_s.p, 2, mom));
+  if (inertia) MA_HIP_TRY(columns_to_host(e, e->eb.mom2_s.p, 3, inertia));
   return MA_OK;
 }
 

@@ -637,6 +637,7 @@ __device__ inline bool clip_tri_edge(const Poly &P, Poly &Q, double ax, double a
 // cubic integrands).
 struct FragSums {
   double mass = 0.0, intg = 0.0, momx = 0.0, momy = 0.0;
+  double sxx = 0.0, syy = 0.0, sxy = 0.0;  // second moments (MODE 3)
 };
 template <int MODE>
 __device__ inline void frag_integrals(const double *qx, const double *qy, int m,
@@ -662,6 +663,7 @@ __device__ inline void frag_integrals(const double *qx, const double *qy, int m,
     AC6 qp;
     ac6_points(x0, y0, x1, y1, x2, y2, qp);
     double s1 = 0.0, s9 = 0.0, m1x = 0.0, m9x = 0.0, m1y = 0.0, m9y = 0.0;
+    double q1[3] = {0.0, 0.0, 0.0}, q9[3] = {0.0, 0.0, 0.0};  // MODE 3: xx, yy, xy
     double c1[3] = {0.0, 0.0, 0.0}, c9[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int p = 0; p < 6; p++) {
@@ -672,6 +674,14 @@ __device__ inline void frag_integrals(const double *qx, const double *qy, int m,
         for (int v = 0; v < 3; v++) {
           const double f = c * (la[v] * qp.px[p] + lb[v] * qp.py[p] + lc[v]);
           if (p < 3) c1[v] += f; else c9[v] += f;
+        }
+      }
+      if (MODE == 3) {
+        const double fx = rp * qp.px[p], fy = rp * qp.py[p];
+        const double t[3] = {fx * qp.px[p], fy * qp.py[p], fx * qp.py[p]};
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+          if (p < 3) q1[v] += t[v]; else q9[v] += t[v];
         }
       }
       if (MODE == 0) {
@@ -693,6 +703,11 @@ __device__ inline void frag_integrals(const double *qx, const double *qy, int m,
       S.momx += area * (m1x * (1.0 / 30.0) + m9x * (9.0 / 30.0));
       S.momy += area * (m1y * (1.0 / 30.0) + m9y * (9.0 / 30.0));
     }
+    if (MODE == 3) {
+      S.sxx += area * (q1[0] * (1.0 / 30.0) + q9[0] * (9.0 / 30.0));
+      S.syy += area * (q1[1] * (1.0 / 30.0) + q9[1] * (9.0 / 30.0));
+      S.sxy += area * (q1[2] * (1.0 / 30.0) + q9[2] * (9.0 / 30.0));
+    }
     if (MODE == 2) {
 #pragma unroll
       for (int v = 0; v < 3; v++) dv[v] += area * (c1[v] * (1.0 / 30.0) + c9[v] * (9.0 / 30.0));
@@ -704,7 +719,8 @@ __device__ inline void frag_integrals(const double *qx, const double *qy, int m,
 // memory) for the cells listed in `list`: the rare cells that overflow the
 // register/LDS fast path.
 // MODE 0: Kantorovich, 1: first moments, 2: first moments + density-value
-// adjoint (integrals of (|x - y_k|^2 - w_k) phi_v, accumulated into dval).
+// adjoint (integrals of (|x - y_k|^2 - w_k) phi_v, accumulated into dval),
+// 3: first and second moments (include/MA/lloyd.hpp:71-123).
 template <int MODE>
 __global__ void __launch_bounds__(kBlock) k_cells_general(
     const int32_t *__restrict__ list, const int32_t *__restrict__ nlist,
@@ -713,8 +729,8 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
     const int32_t *__restrict__ cand_start, const int32_t *__restrict__ cand,
     const int32_t *__restrict__ tbin_start, const int32_t *__restrict__ tbin_items,
     const TriData *__restrict__ tri, double *mass_out, double *intg_out, double *fval_part,
-    double *mom_out, int32_t *row_cnt, int32_t *row_col, double *row_val, int cap_row,
-    const int32_t *__restrict__ triv, double *dval, int32_t *hlist, int force_huge,
+    double *mom_out, double *mom2_out, int32_t *row_cnt, int32_t *row_col, double *row_val,
+    int cap_row, const int32_t *__restrict__ triv, double *dval, int32_t *hlist, int force_huge,
     int32_t *flags, unsigned long long *counters) {
   const int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (idx >= *nlist) return;
@@ -928,8 +944,15 @@ __global__ void __launch_bounds__(kBlock) k_cells_general(
   } else {
     mass_out[k] = S.mass;
     // mode 2 keeps the moments about the Dirac
-    mom_out[2 * k] = S.momx + (MODE == 1 ? yx * S.mass : 0.0);
-    mom_out[2 * k + 1] = S.momy + (MODE == 1 ? yy * S.mass : 0.0);
+    // modes 1 and 3 return absolute moments, mode 2 keeps them about the Dirac
+    const bool absm = MODE == 1 || MODE == 3;
+    mom_out[2 * k] = S.momx + (absm ? yx * S.mass : 0.0);
+    mom_out[2 * k + 1] = S.momy + (absm ? yy * S.mass : 0.0);
+    if (MODE == 3) {
+      mom2_out[3 * k] = S.sxx + 2.0 * yx * S.momx + yx * yx * S.mass;
+      mom2_out[3 * k + 1] = S.syy + 2.0 * yy * S.momy + yy * yy * S.mass;
+      mom2_out[3 * k + 2] = S.sxy + yx * S.momy + yy * S.momx + yx * yy * S.mass;
+    }
   }
   if (overflow) atomicOr(&flags[0], 1);
   atomicAdd(&counters[0], (unsigned long long)nfrag);
@@ -1052,7 +1075,8 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
     const int32_t *__restrict__ rect, const int32_t *__restrict__ cand_start,
     const int32_t *__restrict__ cand, const int32_t *__restrict__ tbin_start,
     const int32_t *__restrict__ tbin_items, const TriData *__restrict__ tri, double *mass_out,
-    double *intg_out, double *fval_part, double *mom_out, int32_t *row_cnt, int32_t *row_col,
+    double *intg_out, double *fval_part, double *mom_out, double *mom2_out, int32_t *row_cnt,
+    int32_t *row_col,
     double *row_val, int cap_row, int32_t *hrow_col, double *hrow_val,
     const int32_t *__restrict__ triv, double *dval, int32_t *flags,
     unsigned long long *counters) {
@@ -1366,6 +1390,12 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
   const double intg = huge_reduce<0>(S.intg, dtmp);
   const double momx = huge_reduce<0>(S.momx, dtmp);
   const double momy = huge_reduce<0>(S.momy, dtmp);
+  double sxx = 0.0, syy = 0.0, sxy = 0.0;
+  if (MODE == 3) {
+    sxx = huge_reduce<0>(S.sxx, dtmp);
+    syy = huge_reduce<0>(S.syy, dtmp);
+    sxy = huge_reduce<0>(S.sxy, dtmp);
+  }
   const int nfr = (int)huge_reduce<0>((double)nfrag, dtmp);
   if (__syncthreads_or(overflow)) {
     if (tid == 0) atomicOr(&flags[0], 2);  // fragment capacity
@@ -1459,8 +1489,14 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
     }
   } else if (tid == 0) {
     mass_out[k] = mass;
-    mom_out[2 * k] = momx + (MODE == 1 ? yx * mass : 0.0);
-    mom_out[2 * k + 1] = momy + (MODE == 1 ? yy * mass : 0.0);
+    const bool absm = MODE == 1 || MODE == 3;
+    mom_out[2 * k] = momx + (absm ? yx * mass : 0.0);
+    mom_out[2 * k + 1] = momy + (absm ? yy * mass : 0.0);
+    if (MODE == 3) {
+      mom2_out[3 * k] = sxx + 2.0 * yx * momx + yx * yx * mass;
+      mom2_out[3 * k + 1] = syy + 2.0 * yy * momy + yy * yy * mass;
+      mom2_out[3 * k + 2] = sxy + yx * momy + yy * momx + yx * yy * mass;
+    }
   }
   if (tid == 0) {
     atomicAdd(&counters[0], (unsigned long long)nfr);
@@ -1632,11 +1668,11 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
     hl = B.hlist.p;
     MA_HIP_TRY(hipMemsetAsync(B.hlist.p + n, 0, sizeof(int32_t), e->stream));
     auto kern = mode == 0 ? k_cells_general<0> : mode == 1 ? k_cells_general<1>
-                                                            : k_cells_general<2>;
+                : mode == 2 ? k_cells_general<2> : k_cells_general<3>;
     hipLaunchKernelGGL(kern, dim3(nblocks(ngen)), dim3(kBlock), 0, e->stream, B.glist.p,
                        B.glist.p + n, g, n, e->pos.p, B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p,
                        B.cand.p, e->tbin_start.p, e->tbin_items.p, e->tri.p,
-                       B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p, B.row_cnt.p,
+                       B.mass_s.p, B.intg_s.p, B.fval_part.p, B.mom_s.p, B.mom2_s.p, B.row_cnt.p,
                        B.row_col.p, B.row_val.p, e->cap_row, e->triv.p, B.dval.p, B.hlist.p,
                        (int)(e->force_general == 2), B.flags.p, B.counters.p);
     MA_HIP_TRY(hipMemcpyAsync(&nhuge, B.hlist.p + n, sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -1649,11 +1685,12 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
       MA_HIP_TRY(B.hrow_col.ensure((size_t)kLongRows * kHugeRow));
       MA_HIP_TRY(B.hrow_val.ensure((size_t)kLongRows * kHugeRow));
     }
-    auto kern = mode == 0 ? k_cell_huge<0> : mode == 1 ? k_cell_huge<1> : k_cell_huge<2>;
+    auto kern = mode == 0 ? k_cell_huge<0> : mode == 1 ? k_cell_huge<1>
+                : mode == 2 ? k_cell_huge<2> : k_cell_huge<3>;
     hipLaunchKernelGGL(kern, dim3(nhuge), dim3(kHugeBlock), 0, e->stream, hl, g, n,
                        e->pos.p, B.w_s.p, e->perm.p, B.rect.p, B.cand_cnt.p, B.cand.p,
                        e->tbin_start.p, e->tbin_items.p, e->tri.p, B.mass_s.p, B.intg_s.p,
-                       B.fval_part.p, B.mom_s.p, B.row_cnt.p, B.row_col.p, B.row_val.p,
+                       B.fval_part.p, B.mom_s.p, B.mom2_s.p, B.row_cnt.p, B.row_col.p, B.row_val.p,
                        e->cap_row, B.hrow_col.p, B.hrow_val.p, e->triv.p, B.dval.p, B.flags.p,
                        B.counters.p);
   }

@@ -435,6 +435,17 @@ static void f_moment(double x, double y, const void *c, double *o) {
   o[1] = r * x;
   o[2] = r * y;
 }
+/* (rho, rho x, rho y, rho x^2, rho y^2, rho x y)  (lloyd.hpp:101-112) */
+static void f_moment2(double x, double y, const void *c, double *o) {
+  const kctx *k = (const kctx *)c;
+  double r = rho_at(k->t, x, y);
+  o[0] = r;
+  o[1] = r * x;
+  o[2] = r * y;
+  o[3] = r * x * x;
+  o[4] = r * y * y;
+  o[5] = r * x * y;
+}
 
 typedef struct {
   const double *c;
@@ -746,13 +757,14 @@ static void vertex_point(const tictx *c, const pedge *e1, const pedge *e2, doubl
 }
 
 /* ====================================================================== */
-/* Driver: kantorovich (kantorovich.hpp:35-141) / first_moment.            */
+/* Driver: kantorovich (kantorovich.hpp:35-141) / first_moment (mode 1,   */
+/* lloyd.hpp:30-69) / second_moment (mode 2, lloyd.hpp:71-123).           */
 /* ====================================================================== */
 struct orc_result {
   int status;
   int64_t n;
   double fval;
-  double *g, *mom;
+  double *g, *mom, *mom2;
   int64_t nt, cap;
   int32_t *ti, *tj;
   double *tv;
@@ -787,6 +799,7 @@ orc_result *orc_run(int mode, int64_t nv, const double *vxy, const double *vval,
   R->n = n;
   R->g = calloc((size_t)(n > 0 ? n : 1), sizeof(double));
   R->mom = calloc((size_t)(2 * (n > 0 ? n : 1)), sizeof(double));
+  R->mom2 = calloc((size_t)(3 * (n > 0 ? n : 1)), sizeof(double));
   g_npred = g_nexact = 0;
   if (nv <= 0 || nt <= 0 || n <= 0) return R;
 
@@ -968,6 +981,21 @@ orc_result *orc_run(int mode, int64_t nv, const double *vxy, const double *vval,
             }
             fval = fval + warea * w[v] - intg;
             R->g[v] += warea;
+          } else if (mode == 2) {
+            /* second_moment (lloyd.hpp:71-123): all six integrals by the
+               Albrecht-Collatz rule (integrate_3, quadrature.hpp:133-143) */
+            kctx kc = {t, yvx, yvy};
+            for (int k = 1; k < m - 1; k++) {
+              double r6[6];
+              integrate_ac6(px[0], py[0], px[k], py[k], px[k + 1], py[k + 1], f_moment2, &kc,
+                            6, r6);
+              R->g[v] += r6[0];
+              R->mom[2 * v] += r6[1];
+              R->mom[2 * v + 1] += r6[2];
+              R->mom2[3 * v] += r6[3];
+              R->mom2[3 * v + 1] += r6[4];
+              R->mom2[3 * v + 2] += r6[5];
+            }
           } else {
             /* first_moment (lloyd.hpp:30-69) */
             kctx kc = {t, yvx, yvy};
@@ -1011,10 +1039,14 @@ void orc_copy(const orc_result *r, double *fval, double *g, double *mom, int32_t
   if (tj && r->nt) memcpy(tj, r->tj, (size_t)r->nt * sizeof(int32_t));
   if (tv && r->nt) memcpy(tv, r->tv, (size_t)r->nt * sizeof(double));
 }
+void orc_copy_mom2(const orc_result *r, double *mom2) {
+  if (mom2 && r->n > 0) memcpy(mom2, r->mom2, (size_t)(3 * r->n) * sizeof(double));
+}
 void orc_free(orc_result *r) {
   if (!r) return;
   free(r->g);
   free(r->mom);
+  free(r->mom2);
   free(r->ti);
   free(r->tj);
   free(r->tv);

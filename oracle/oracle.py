@@ -48,6 +48,8 @@ def _lib():
         lib.orc_copy.argtypes = [P, P, P, P, P, P, P, P]
         lib.orc_free.restype = None
         lib.orc_free.argtypes = [P]
+        lib.orc_copy_mom2.restype = None
+        lib.orc_copy_mom2.argtypes = [P, P]
         for nm in ("orc_side1", "orc_side1_exact"):
             f = getattr(lib, nm)
             f.restype = ctypes.c_int
@@ -116,9 +118,14 @@ def _run(mode, mesh, X, w):
         tv = np.zeros(max(nt, 1))
         stats = np.zeros(6, np.int64)
         lib.orc_copy(r, _ptr(fval), _ptr(g), _ptr(mom), _ptr(ti), _ptr(tj), _ptr(tv), _ptr(stats))
+        mom2 = np.zeros((n, 3))
+        if mode == 2:
+            lib.orc_copy_mom2(r, _ptr(mom2))
     finally:
         lib.orc_free(r)
     s = OracleStats(*[int(v) for v in stats], seconds=dt)
+    if mode == 2:
+        return float(fval[0]), g, (mom, mom2), (ti[:nt], tj[:nt], tv[:nt]), s
     return float(fval[0]), g, mom, (ti[:nt], tj[:nt], tv[:nt]), s
 
 
@@ -144,6 +151,13 @@ def first_moment(mesh, X, w):
     """Cell masses and first moments (``include/MA/lloyd.hpp:30-69``)."""
     _, m, mom, _, _ = _run(1, mesh, X, w)
     return m, mom
+
+
+def second_moment(mesh, X, w):
+    """Cell masses, first moments and second moments (int rho x^2, int rho y^2,
+    int rho x y), ``include/MA/lloyd.hpp:71-123``."""
+    _, m, (mom, mom2), _, _ = _run(2, mesh, X, w)
+    return m, mom, mom2
 
 
 def lloyd(mesh, X, w):

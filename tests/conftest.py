@@ -34,7 +34,8 @@ def load_golden(name):
         H[i, j] += v
         pattern.add((i, j))
     return dict(name=name, mesh=mesh, X=X, w=w, fval=e["fval"], mass=np.array(e["mass"]),
-                intg=np.array(e["intg"]), mom=np.array(e["mom"]), H=H, pattern=pattern)
+                intg=np.array(e["intg"]), mom=np.array(e["mom"]), H=H, pattern=pattern,
+                inertia=np.array(e["inertia"]))
 
 
 GOLDEN_CASES = sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json"))

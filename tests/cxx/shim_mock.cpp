@@ -5,7 +5,7 @@
 // Input (text file): nv, then nv lines "x y value"; nt, then nt lines
 // "i j k"; n, then n lines "x y nu".  Output: one JSON object with the
 // results of MA::kantorovich (w = 0), MA::ot_solve, MA::solve_laplacian_matrix
-// and MA::lloyd.
+// MA::lloyd and MA::second_moment.
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -163,6 +163,9 @@ int main(int argc, char **argv) {
   mock::Vector m;
   mock::Matrix cen;
   MA::lloyd(T, F, X, x, m, cen);
+  mock::Vector m2;
+  mock::Matrix mom, inertia;
+  MA::second_moment(T, F, X, x, m2, mom, inertia);
 
   std::printf("{\"fval\": %.17g, \"niter\": %zu, \"neval\": %zu,\n", fval, st.niter, st.neval);
   auto vec = [&](const char *name, const mock::Vector &v, bool last = false) {
@@ -177,6 +180,10 @@ int main(int argc, char **argv) {
   std::printf("\"centroids\": [");
   for (int64_t i = 0; i < n; i++)
     std::printf("%s[%.17g, %.17g]", i ? ", " : "", cen(i, 0), cen(i, 1));
+  std::printf("],\n\"inertia\": [");
+  for (int64_t i = 0; i < n; i++)
+    std::printf("%s[%.17g, %.17g, %.17g]", i ? ", " : "", inertia(i, 0), inertia(i, 1),
+                inertia(i, 2));
   std::printf("],\n\"hessian\": [");
   bool first = true;
   for (int64_t j = 0; j < n; j++)

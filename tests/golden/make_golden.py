@@ -5,7 +5,8 @@ triangle is clipped by the half-planes {pi_i < pi_j} of ALL other Diracs in
 exact rational arithmetic (``fractions.Fraction``), with the reference's strict
 tie rule (a point with pi_i == pi_j counts as outside,
 ``include/MA/predicates.hpp:85-86``).  Masses, the value and the first moments
-are exact rationals (the centroid rule and the Albrecht-Collatz rule are exact
+(and the second moments int rho x^2, y^2, xy of include/MA/lloyd.hpp:71-123) are exact
+rationals (the centroid rule and the Albrecht-Collatz rule are exact
 for P1 / cubic integrands, ``include/MA/quadrature.hpp:23-42,69-77``); the
 Hessian entries -rho(mid) |e| / (2 |y_i - y_j|) (``include/MA/kantorovich.hpp:
 109-122``) involve square roots and are evaluated with mpmath at 50 digits.
@@ -94,6 +95,9 @@ def solve(vxy, vval, tri, X, w):
     intg = [Fr(0)] * n
     mx = [Fr(0)] * n
     my = [Fr(0)] * n
+    sxx = [Fr(0)] * n
+    syy = [Fr(0)] * n
+    sxy = [Fr(0)] * n
     hess = {}
     for t in tri:
         p = [V[k] for k in t]
@@ -126,6 +130,9 @@ def solve(vxy, vval, tri, X, w):
                 intg[i] += ac6(a, b, c, lambda q: rho(q) * ((q[0] - yi[0]) ** 2 + (q[1] - yi[1]) ** 2))
                 mx[i] += ac6(a, b, c, lambda q: rho(q) * q[0])
                 my[i] += ac6(a, b, c, lambda q: rho(q) * q[1])
+                sxx[i] += ac6(a, b, c, lambda q: rho(q) * q[0] * q[0])
+                syy[i] += ac6(a, b, c, lambda q: rho(q) * q[1] * q[1])
+                sxy[i] += ac6(a, b, c, lambda q: rho(q) * q[0] * q[1])
             for k in range(len(poly)):
                 j = labs[k]
                 if j < 0:
@@ -145,6 +152,7 @@ def solve(vxy, vval, tri, X, w):
         mass=[float(m) for m in mass],
         intg=[float(v) for v in intg],
         mom=[[float(mx[i]), float(my[i])] for i in range(n)],
+        inertia=[[float(sxx[i]), float(syy[i]), float(sxy[i])] for i in range(n)],
         total=float(sum(mass)),
         hessian=[[i, j, float(v)] for (i, j), v in sorted(hess.items())],
     )

@@ -66,3 +66,5 @@ def test_shim_matches_c_abi(engine, tmp_path):
     assert out["niter"] == st.niter and out["neval"] == st.neval
     m, c = engine.lloyd(w)
     assert np.max(np.abs(np.array(out["centroids"]) - c)) <= 1e-12
+    _, _, inertia = engine.second_moment(w)
+    assert np.max(np.abs(np.array(out["inertia"]) - inertia)) <= 1e-14

@@ -38,8 +38,12 @@ def test_golden_exact(engine, name, path):
     try:
         fval, g, H = engine.kantorovich(c["w"])
         m, mom = engine.first_moment(c["w"])
+        m2, mom2, inertia = engine.second_moment(c["w"])
     finally:
         engine.set_option("force_general", 0)
+    assert _rel(m2, c["mass"]) <= 1e-12
+    assert np.abs(mom2 - c["mom"]).max() <= 1e-12 * max(np.abs(c["mom"]).max(), 1e-300)
+    assert np.abs(inertia - c["inertia"]).max() <= 1e-12 * max(np.abs(c["inertia"]).max(), 1e-300)
     assert _rel(g, c["mass"]) <= 1e-12
     assert abs(fval - c["fval"]) <= 1e-12 * max(abs(c["fval"]), 1e-300)
     if c["H"].any():
@@ -65,6 +69,24 @@ def random_problem(seed):
     scale = float(((hi - lo) ** 2).sum()) / N
     w = rng.uniform(-1, 1, N) * scale * float(rng.choice([0.0, 0.1, 1.0, 5.0]))
     return mesh, X, w
+
+
+@pytest.mark.parametrize("path", list(PATHS))
+@pytest.mark.parametrize("seed", [0, 3, 7, 10])
+def test_second_moment_vs_oracle(engine, seed, path):
+    """MA::second_moment (include/MA/lloyd.hpp:71-123) on random problems."""
+    mesh, X, w = random_problem(seed)
+    print(f"seed {seed} N={X.shape[0]} path {path}", flush=True)
+    engine.set_option("force_general", PATHS[path])
+    engine.set_mesh(mesh).set_diracs(X)
+    try:
+        m, mom, inertia = engine.second_moment(w)
+    finally:
+        engine.set_option("force_general", 0)
+    om, omom, oin = oracle.second_moment(mesh, X, w)
+    assert _rel(m, om) <= TOL
+    assert np.abs(mom - omom).max() <= TOL * np.abs(omom).max()
+    assert np.abs(inertia - oin).max() <= TOL * np.abs(oin).max()
 
 
 @pytest.mark.parametrize("path", list(PATHS))

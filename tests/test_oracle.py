@@ -36,6 +36,31 @@ def test_oracle_vs_golden(name):
         assert _rel(H.toarray(), c["H"]) <= 1e-13
     m, mom = oracle.first_moment(c["mesh"], c["X"], c["w"])
     assert np.abs(mom - c["mom"]).max() <= 1e-13 * max(np.abs(c["mom"]).max(), 1e-300)
+    m2, mom2, inertia = oracle.second_moment(c["mesh"], c["X"], c["w"])
+    assert _rel(m2, c["mass"]) <= 1e-13
+    assert np.abs(mom2 - c["mom"]).max() <= 1e-13 * max(np.abs(c["mom"]).max(), 1e-300)
+    assert np.abs(inertia - c["inertia"]).max() <= 1e-13 * max(np.abs(c["inertia"]).max(), 1e-300)
+
+
+@pytest.mark.parametrize("k", [1, 2, 4])
+def test_second_moment_lattice_closed_form(k):
+    """Uniform density on [0,1]^2, k x k lattice, w = 0: the cell of the Dirac
+    at (cx, cy) is the square of side a = 1/k around it, so int x^2 =
+    a^2 (cx^2 + a^2/12), int x y = a^2 cx cy.  A swapped x/y or a dropped
+    a^2/12 term fails this."""
+    vxy, tri = configs.grid_mesh(2 * k + 1)
+    mesh = configs.Mesh(vxy, np.ones(vxy.shape[0]), tri)
+    c = (np.arange(k) + 0.5) / k
+    X = np.array([[x, y] for x in c for y in c])  # off the diagonal, x != y: a swap shows
+    a = 1.0 / k
+    m, mom, inertia = oracle.second_moment(mesh, X, np.zeros(k * k))
+    ex = a * a * (X[:, 0] ** 2 + a * a / 12)
+    ey = a * a * (X[:, 1] ** 2 + a * a / 12)
+    exy = a * a * X[:, 0] * X[:, 1]
+    assert np.abs(inertia[:, 0] - ex).max() <= 1e-15
+    assert np.abs(inertia[:, 1] - ey).max() <= 1e-15
+    assert np.abs(inertia[:, 2] - exy).max() <= 1e-15
+    assert np.abs(mom - a * a * X).max() <= 1e-15
 
 
 # --------------------------------------------------------------------------
