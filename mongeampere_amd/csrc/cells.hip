@@ -74,9 +74,8 @@ struct CellArgs {
   int32_t *tcount;   // candidate triangles per cell
   double *mass, *intg, *mom;  // mom: local first moments, interleaved (x, y)
   double *mom2;      // local second moments (xx, yy, xy) per Dirac (mode 3)
-  double *eacc;      // edge integrals int rho ds, [v * n + k]
+  double *eacc;      // edge integrals int rho ds, [v * n + k]; -0.0 until an edge is seen
   float *eaccf;      // FP32 edge integrals (mixed mode)
-  int32_t *ehit;     // edge seen in a fragment
   const int32_t *triv;  // triangle vertex ids (counter-clockwise, as in tri)
   double *dval;      // adjoint w.r.t. vertex density values (mode 2)
   int32_t *flags;
@@ -315,8 +314,7 @@ __device__ inline void write_cell(const CellArgs &a, int64_t k, double yx, doubl
     px[v * N + k] = x;
     py[v * N + k] = y;
     a.plab[v * N + k] = L.sl[v][L.t];
-    eacc[v * N + k] = R(0);
-    a.ehit[v * N + k] = 0;
+    eacc[v * N + k] = R(-0.0);  // "not seen": any contribution (>= 0) clears the sign
     bx0 = fmin(bx0, (double)x);
     bx1 = fmax(bx1, (double)x);
     by0 = fmin(by0, (double)y);
@@ -380,6 +378,12 @@ __device__ inline bool certify_levels(const Grid &g, const double *pyr3, const P
     x0 = bx0; x1 = bx1; y0 = by0; y1 = by1;
   }
   return true;
+}
+
+// -0.0 exactly (an edge integral no fragment has touched)
+__device__ inline bool is_neg_zero(float x) { return __float_as_uint(x) == 0x80000000u; }
+__device__ inline bool is_neg_zero(double x) {
+  return (unsigned long long)__double_as_longlong(x) == 0x8000000000000000ull;
 }
 
 // Counts the lanes of the wave with flag set: one atomic per wave, issued by
@@ -731,11 +735,9 @@ struct Integrator {
     return area > Tol<R>::area * ((xhi - xlo) + (yhi - ylo)) * cmax;
   }
   R *eacc;
-  int32_t *ehit;
   int64_t N, k;
   __device__ void commit(int lab, R r) {
     atomicAdd(&eacc[lab * N + k], r);
-    ehit[lab * N + k] = 1;
   }
   __device__ void edge(const SV<R> &a, const SV<R> &b) {
     if (MODE != 0 || a.lab < 0) return;
@@ -856,12 +858,15 @@ template <int MODE, class R>
 __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
                                               const int32_t *__restrict__ pair_cell,
                                               const int32_t *__restrict__ pair_tri) {
-  const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (p >= npairs) return;
+  // every lane stays to the end (wave-level sums below); lanes past the last
+  // pair take the last pair with an empty polygon, which yields nothing
+  const int64_t p0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool valid = p0 < npairs;
+  const int64_t p = valid ? p0 : npairs - 1;
   const int64_t k = pair_cell[p];
   const TriData t = a.tri[pair_tri[p]];
   const int64_t N = a.n;
-  const int n = a.pn[k];
+  const int n = valid ? a.pn[k] : 0;
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1];
   // triangle in coordinates local to the Dirac (FP64, then rounded once)
   const R X0 = (R)(t.x[0] - yx), Y0 = (R)(t.y[0] - yy), X1 = (R)(t.x[1] - yx);
@@ -885,7 +890,6 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   I.xhi = I.yhi = R(-kInf);
   I.cmax = R(0);
   I.eacc = CellBuf<R>::eacc(a);
-  I.ehit = a.ehit;
   I.N = N;
   I.k = k;
   if (MODE == 2) {
@@ -932,25 +936,73 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
     if (m3 > 1) I.push(r1);
     I.finish();
   }
-  const bool counted = I.cnt >= 3 && I.positive();
-  if (counted) {
-    if (MODE == 0) {
-      atomicAdd(&a.mass[k], (double)I.mass - (double)I.cmass);
-      atomicAdd(&a.intg[k], (double)I.intg - (double)I.cintg);
-    } else {
-      atomicAdd(&a.mass[k], (double)I.mass);
-      atomicAdd(&a.mom[2 * k], (double)I.mx);
-      atomicAdd(&a.mom[2 * k + 1], (double)I.my);
-    }
-    if (MODE == 3) {
-      atomicAdd(&a.mom2[3 * k], (double)I.sxx);
-      atomicAdd(&a.mom2[3 * k + 1], (double)I.syy);
-      atomicAdd(&a.mom2[3 * k + 2], (double)I.sxy);
-    }
-    if (MODE == 2) {
-      const int64_t ti = pair_tri[p];
+  const bool counted = valid && I.cnt >= 3 && I.positive();
+  if (counted && MODE == 2) {
+    const int64_t ti = pair_tri[p];
 #pragma unroll
-      for (int v = 0; v < 3; v++) atomicAdd(&a.dval[a.triv[3 * ti + v]], I.dv[v]);
+    for (int v = 0; v < 3; v++) atomicAdd(&a.dval[a.triv[3 * ti + v]], I.dv[v]);
+  }
+  // Per-cell sums: pairs come sorted by cell, so the threads of a block
+  // holding the same cell are contiguous.  Each thread's contributions go to
+  // LDS; the last thread of each run sums its run and issues one atomic per
+  // quantity (instead of one per fragment).  Every thread reaches the
+  // barriers (no early return above).
+  constexpr int kQ = MODE == 0 ? 2 : MODE == 3 ? 6 : 3;
+  __shared__ int skey[256];
+  __shared__ double sval[kQ][256];
+  {
+    double q[kQ];
+    if (MODE == 0) {
+      q[0] = (double)I.mass - (double)I.cmass;
+      q[1] = (double)I.intg - (double)I.cintg;
+    } else {
+      q[0] = (double)I.mass;
+      q[1] = (double)I.mx;
+      q[2] = (double)I.my;
+      if (MODE == 3) {
+        q[3 % kQ] = (double)I.sxx;
+        q[4 % kQ] = (double)I.syy;
+        q[5 % kQ] = (double)I.sxy;
+      }
+    }
+    const int t = (int)threadIdx.x;
+    skey[t] = counted ? (int)k : (valid ? -2 - (int)(k & 0x3fffffff) : -1);
+#pragma unroll
+    for (int c = 0; c < kQ; c++) sval[c][t] = counted ? q[c] : 0.0;
+    __syncthreads();
+    // runs are keyed by cell; uncounted fragments keep their cell's run
+    // boundaries (key -2 - k) but contribute nothing
+    const int mykey = valid ? (int)k : -1;
+    auto cell_of = [&](int j) {
+      const int kk = skey[j];
+      return kk >= 0 ? kk : (kk == -1 ? -1 : -2 - kk);
+    };
+    const bool last = t == 255 || cell_of(t + 1) != mykey;
+    if (valid && last) {
+      double acc[kQ];
+#pragma unroll
+      for (int c = 0; c < kQ; c++) acc[c] = 0.0;
+      bool any = false;
+      for (int j = t; j >= 0 && cell_of(j) == mykey; j--) {
+        any |= skey[j] >= 0;
+#pragma unroll
+        for (int c = 0; c < kQ; c++) acc[c] += sval[c][j];
+      }
+      if (any) {
+        if (MODE == 0) {
+          atomicAdd(&a.mass[k], acc[0]);
+          atomicAdd(&a.intg[k], acc[1]);
+        } else {
+          atomicAdd(&a.mass[k], acc[0]);
+          atomicAdd(&a.mom[2 * k], acc[1]);
+          atomicAdd(&a.mom[2 * k + 1], acc[2]);
+          if (MODE == 3) {
+            atomicAdd(&a.mom2[3 * k], acc[3 % kQ]);
+            atomicAdd(&a.mom2[3 * k + 1], acc[4 % kQ]);
+            atomicAdd(&a.mom2[3 * k + 2], acc[5 % kQ]);
+          }
+        }
+      }
     }
   }
   wave_count(&a.counters[0], counted);
@@ -974,7 +1026,7 @@ __global__ void k_rows(CellArgs a, double *fval_part, int32_t *row_cnt, int32_t 
     double diag = 0.0;
     for (int v = 0; v < n; v++) {
       const int32_t j = a.plab[v * N + k];
-      if (j < 0 || !a.ehit[v * N + k]) continue;
+      if (j < 0 || is_neg_zero(eacc[v * N + k])) continue;  // edge met by no fragment
       const double dx = yx - a.pos[2 * j], dy = yy - a.pos[2 * j + 1];
       const double val = (double)eacc[v * N + k] / (2.0 * sqrt(dx * dx + dy * dy));
       diag += val;
@@ -1084,7 +1136,6 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
     MA_HIP_TRY(B.eacc.ensure((size_t)kMaxFast * n));
   }
   MA_HIP_TRY(B.plab.ensure((size_t)kMaxFast * n));
-  MA_HIP_TRY(B.ehit.ensure((size_t)kMaxFast * n));
   MA_HIP_TRY(B.pn.ensure((size_t)n));
   MA_HIP_TRY(B.bbox.ensure((size_t)(4 * n)));
   MA_HIP_TRY(B.tcount.ensure((size_t)n + 1));
@@ -1118,7 +1169,6 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   a.mom2 = mode == 3 ? B.mom2_s.p : nullptr;
   a.eacc = B.eacc.p;
   a.eaccf = B.eaccf.p;
-  a.ehit = B.ehit.p;
   a.flags = B.flags.p;
   a.counters = B.counters.p;
   a.force_general = e->force_general;

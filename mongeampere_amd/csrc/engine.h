@@ -119,7 +119,7 @@ struct EvalBuffers {
   // register/LDS fast path
   DBuf<double> px, py, eacc, bbox;
   DBuf<float> pxf, pyf, eaccf;   // FP32 polygons and edge integrals (mixed mode)
-  DBuf<int32_t> plab, ehit, pn, tcount, poff, pair_cell, pair_tri, glist;
+  DBuf<int32_t> plab, pn, tcount, poff, pair_cell, pair_tri, glist;
   // cells beyond the general kernel (list + count at [n]) and their long rows
   DBuf<int32_t> hlist, hrow_col;
   DBuf<double> hrow_val;
