@@ -139,10 +139,13 @@ __global__ void k_amg_smooth(int64_t n, const int32_t *ptr, const int32_t *col,
   const double ax = row_dot(ptr, col, val, x, i, i < n, lane);
   if (i < n && lane == 0) y[i] = x[i] + kOmega * (b[i] - ax) * dinv[i];
 }
-// bc[I] = sum over the aggregate of (b - A x): kAgg rows x kRowLanes lanes
+// bc[I] = sum over the aggregate of (b - A x): kAgg rows x kRowLanes lanes;
+// with dinv_c, also the coarse level's first smoothing sweep from zero,
+// yc = omega bc / Dc (saves that level a launch)
 __global__ void k_amg_restrict(int64_t nc, int64_t n, const int32_t *ptr, const int32_t *col,
                                const double *val, const double *b, const double *x,
-                               double *bc, const double *done) {
+                               double *bc, const double *dinv_c, double *yc,
+                               const double *done) {
   if (*done != 0.0) return;
   constexpr int kGroup = kAgg * kRowLanes;
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -153,13 +156,33 @@ __global__ void k_amg_restrict(int64_t nc, int64_t n, const int32_t *ptr, const 
   const double ax = row_dot(ptr, col, val, x, i, valid, lane % kRowLanes);
   double s = (valid && lane % kRowLanes == 0) ? b[i] - ax : 0.0;
   for (int o = kRowLanes; o < kGroup; o <<= 1) s += __shfl_xor(s, o, kGroup);
-  if (I < nc && lane == 0) bc[I] = s;
+  if (I < nc && lane == 0) {
+    bc[I] = s;
+    if (dinv_c) yc[I] = kOmega * s * dinv_c[I];
+  }
 }
-// x += corr * xc[i / kAgg]
-__global__ void k_amg_prolong(int64_t n, const double *xc, double *x, const double *done) {
+// First post-smoothing sweep fused with the prolongation: with
+// x' = x + corr * xc[. / kAgg] (the corrected iterate, formed on the fly),
+// y = x' + omega (b - A x') / D
+__global__ void k_amg_smooth_prolong(int64_t n, const int32_t *ptr, const int32_t *col,
+                                     const double *val, const double *dinv, const double *b,
+                                     const double *x, const double *xc, double *y,
+                                     const double *done) {
   if (*done != 0.0) return;
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i < n) x[i] += kCorr * xc[i / kAgg];
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t i = t / kRowLanes;
+  const int lane = (int)(t % kRowLanes);
+  double ax = 0.0;
+  if (i < n)
+    for (int32_t q = ptr[i] + lane; q < ptr[i + 1]; q += kRowLanes) {
+      const int32_t j = col[q];
+      ax += val[q] * (x[j] + kCorr * xc[j / kAgg]);
+    }
+  for (int o = 1; o < kRowLanes; o <<= 1) ax += __shfl_xor(ax, o, kRowLanes);
+  if (i < n && lane == 0) {
+    const double xi = x[i] + kCorr * xc[i / kAgg];
+    y[i] = xi + kOmega * (b[i] - ax) * dinv[i];
+  }
 }
 // Inverse of the coarsest matrix from its Cholesky factor L (row-major,
 // lower): thread j solves L L^T x = e_j, column j of the inverse.
@@ -291,18 +314,21 @@ hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const dou
   const int nl = A.nlev;
   auto ptr_of = [&](int l) { return l == 0 ? A.ptr0 : A.lv[l].ptr.p; };
   auto col_of = [&](int l) { return l == 0 ? A.col0 : A.lv[l].col.p; };
-  // downward
+  // downward (the first sweep of a coarse level comes from the restriction)
   for (int l = 0; l < nl - 1; l++) {
     AmgLevel &L = A.lv[l];
     const double *b = (l == 0) ? r : L.b.p;
     double *x = (l == 0) ? z : L.x.p;
-    hipLaunchKernelGGL(k_amg_smooth0, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, L.dinv.p, b,
-                       L.y.p, done);
+    if (l == 0)
+      hipLaunchKernelGGL(k_amg_smooth0, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, L.dinv.p,
+                         b, L.y.p, done);
     hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
                        ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done);
-    hipLaunchKernelGGL(k_amg_restrict, dim3(nb2(A.lv[l + 1].n * kAgg * kRowLanes)), dim3(kVB2), 0,
-                       e->stream, A.lv[l + 1].n, L.n, ptr_of(l), col_of(l), L.val.p, b, x,
-                       A.lv[l + 1].b.p, done);
+    AmgLevel &Ln = A.lv[l + 1];
+    const bool seed = l + 1 < nl - 1;  // the coarsest level is solved directly
+    hipLaunchKernelGGL(k_amg_restrict, dim3(nb2(Ln.n * kAgg * kRowLanes)), dim3(kVB2), 0,
+                       e->stream, Ln.n, L.n, ptr_of(l), col_of(l), L.val.p, b, x, Ln.b.p,
+                       seed ? Ln.dinv.p : (const double *)nullptr, seed ? Ln.y.p : nullptr, done);
   }
   {
     AmgLevel &Lc = A.lv[nl - 1];
@@ -316,10 +342,9 @@ hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const dou
     AmgLevel &L = A.lv[l];
     const double *b = (l == 0) ? r : L.b.p;
     double *x = (l == 0) ? z : L.x.p;
-    hipLaunchKernelGGL(k_amg_prolong, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n,
-                       A.lv[l + 1].x.p, x, done);
-    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
-                       ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, x, L.y.p, done);
+    hipLaunchKernelGGL(k_amg_smooth_prolong, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0,
+                       e->stream, L.n, ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, x,
+                       A.lv[l + 1].x.p, L.y.p, done);
     hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
                        ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done);
   }
