@@ -74,6 +74,7 @@ struct CellArgs {
   int32_t *tcount;   // candidate triangles per cell
   double *mass, *intg, *mom;  // mom: local first moments, interleaved (x, y)
   double *mom2;      // local second moments (xx, yy, xy) per Dirac (mode 3)
+  int32_t *ovf, *novf;  // pairs left to k_frag_wide, and their count
   double *eacc;      // edge integrals int rho ds, [v * n + k]; -0.0 until an edge is seen
   float *eaccf;      // FP32 edge integrals (mixed mode)
   const int32_t *triv;  // triangle vertex ids (counter-clockwise, as in tri)
@@ -854,28 +855,18 @@ struct Integrator {
   }
 };
 
+// Sets up the three clip stages (one per triangle edge, in coordinates local
+// to the Dirac) and the integrator of pair (cell k, triangle t).
 template <int MODE, class R>
-__global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
-                                              const int32_t *__restrict__ pair_cell,
-                                              const int32_t *__restrict__ pair_tri) {
-  // every lane stays to the end (wave-level sums below); lanes past the last
-  // pair take the last pair with an empty polygon, which yields nothing
-  const int64_t p0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const bool valid = p0 < npairs;
-  const int64_t p = valid ? p0 : npairs - 1;
-  const int64_t k = pair_cell[p];
-  const TriData t = a.tri[pair_tri[p]];
-  const int64_t N = a.n;
-  const int n = valid ? a.pn[k] : 0;
+__device__ inline void frag_setup(const CellArgs &a, int64_t k, const TriData &t, Stage<R> &s1,
+                                  Stage<R> &s2, Stage<R> &s3, Integrator<MODE, R> &I) {
   const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1];
   // triangle in coordinates local to the Dirac (FP64, then rounded once)
   const R X0 = (R)(t.x[0] - yx), Y0 = (R)(t.y[0] - yy), X1 = (R)(t.x[1] - yx);
   const R Y1 = (R)(t.y[1] - yy), X2 = (R)(t.x[2] - yx), Y2 = (R)(t.y[2] - yy);
-  Stage<R> s1, s2, s3;
   s1.init(X0, Y0, X1, Y1);
   s2.init(X1, Y1, X2, Y2);
   s3.init(X2, Y2, X0, Y0);
-  Integrator<MODE, R> I;
   I.cnt = 0;
   I.area = I.mass = I.intg = I.mx = I.my = R(0);
   I.sxx = I.syy = I.sxy = R(0);
@@ -890,7 +881,7 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   I.xhi = I.yhi = R(-kInf);
   I.cmax = R(0);
   I.eacc = CellBuf<R>::eacc(a);
-  I.N = N;
+  I.N = a.n;
   I.k = k;
   if (MODE == 2) {
     // barycentric coordinates of the triangle in coordinates local to the Dirac
@@ -903,12 +894,20 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
     I.wk = a.w[k];
     I.dv[0] = I.dv[1] = I.dv[2] = 0.0;
   }
+}
+
+// Streams the n-vertex cell polygon of k through the three clip stages;
+// sink(v) receives the fragment's vertices in order.
+template <class R, class Sink>
+__device__ inline void frag_clip(const CellArgs &a, int64_t k, int n, Stage<R> &s1,
+                                 Stage<R> &s2, Stage<R> &s3, Sink &&sink) {
+  const int64_t N = a.n;
   const R *px = CellBuf<R>::px(a), *py = CellBuf<R>::py(a);
   SV<R> o0, o1, q0, q1, r0, r1;
   auto to3 = [&](const SV<R> &v) {
     const int m3 = s3.push(v, r0, r1);
-    if (m3 > 0) I.push(r0);
-    if (m3 > 1) I.push(r1);
+    if (m3 > 0) sink(r0);
+    if (m3 > 1) sink(r1);
   };
   auto to2 = [&](const SV<R> &v) {
     const int m2 = s2.push(v, q0, q1);
@@ -924,19 +923,74 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
     if (m1 > 0) to2(o0);
     if (m1 > 1) to2(o1);
   }
-  {
-    const int m1 = s1.flush(o0, o1);
-    if (m1 > 0) to2(o0);
-    if (m1 > 1) to2(o1);
-    const int m2 = s2.flush(q0, q1);
-    if (m2 > 0) to3(q0);
-    if (m2 > 1) to3(q1);
-    const int m3 = s3.flush(r0, r1);
-    if (m3 > 0) I.push(r0);
-    if (m3 > 1) I.push(r1);
+  const int m1 = s1.flush(o0, o1);
+  if (m1 > 0) to2(o0);
+  if (m1 > 1) to2(o1);
+  const int m2 = s2.flush(q0, q1);
+  if (m2 > 0) to3(q0);
+  if (m2 > 1) to3(q1);
+  const int m3 = s3.flush(r0, r1);
+  if (m3 > 0) sink(r0);
+  if (m3 > 1) sink(r1);
+}
+
+// Fragment vertices staged per thread in LDS by k_frag (FP32 / FP64).
+template <class R>
+struct FragCap {
+  static constexpr int value = sizeof(R) == sizeof(float) ? 12 : 15;
+};
+
+template <int MODE, class R>
+__global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
+                                              const int32_t *__restrict__ pair_cell,
+                                              const int32_t *__restrict__ pair_tri) {
+  // every thread stays to the end (block-level sums below); threads past
+  // the last pair take the last pair with an empty polygon (nothing)
+  const int64_t p0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const bool valid = p0 < npairs;
+  const int64_t p = valid ? p0 : npairs - 1;
+  const int64_t k = pair_cell[p];
+  const TriData t = a.tri[pair_tri[p]];
+  const int n = valid ? a.pn[k] : 0;
+  Stage<R> s1, s2, s3;
+  Integrator<MODE, R> I;
+  frag_setup<MODE, R>(a, k, t, s1, s2, s3, I);
+  // The clip pipeline's output vertices are staged in LDS and integrated
+  // afterwards in one flat loop: calling the integrator from inside the
+  // pipeline's data-dependent branches made every wave execute the
+  // quadrature once per possible path.  Fragments longer than the staging
+  // (rare) go to k_frag_wide.  The same LDS holds the per-cell sums at the end.
+  constexpr int kCap = FragCap<R>::value;
+  constexpr int kStageBytes = kCap * 256 * (2 * (int)sizeof(R) + 4);
+  constexpr int kRedBytes = 256 * 4 + 6 * 256 * 8;
+  __shared__ __align__(16) unsigned char lds[kStageBytes > kRedBytes ? kStageBytes : kRedBytes];
+  R *ox = reinterpret_cast<R *>(lds);
+  R *oy = ox + kCap * 256;
+  int *ol = reinterpret_cast<int *>(oy + kCap * 256);
+  const int tt = (int)threadIdx.x;
+  int nout = 0;
+  frag_clip<R>(a, k, n, s1, s2, s3, [&](const SV<R> &v) {
+    if (nout < kCap) {
+      ox[nout * 256 + tt] = v.x;
+      oy[nout * 256 + tt] = v.y;
+      ol[nout * 256 + tt] = v.lab;
+    }
+    nout++;
+  });
+  const bool wide = nout > kCap;
+  if (wide) {
+    a.ovf[atomicAdd(a.novf, 1)] = (int32_t)p;
+  } else {
+    for (int i = 0; i < nout; i++) {
+      SV<R> v;
+      v.x = ox[i * 256 + tt];
+      v.y = oy[i * 256 + tt];
+      v.lab = ol[i * 256 + tt];
+      I.push(v);
+    }
     I.finish();
   }
-  const bool counted = valid && I.cnt >= 3 && I.positive();
+  const bool counted = valid && !wide && I.cnt >= 3 && I.positive();
   if (counted && MODE == 2) {
     const int64_t ti = pair_tri[p];
 #pragma unroll
@@ -948,8 +1002,9 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   // quantity (instead of one per fragment).  Every thread reaches the
   // barriers (no early return above).
   constexpr int kQ = MODE == 0 ? 2 : MODE == 3 ? 6 : 3;
-  __shared__ int skey[256];
-  __shared__ double sval[kQ][256];
+  __syncthreads();  // every thread is done with its staged vertices
+  int *skey = reinterpret_cast<int *>(lds);
+  double (*sval)[256] = reinterpret_cast<double (*)[256]>(lds + 256 * 4);
   {
     double q[kQ];
     if (MODE == 0) {
@@ -1006,6 +1061,43 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
     }
   }
   wave_count(&a.counters[0], counted);
+}
+
+// The pairs whose fragment outgrew k_frag's staging: streaming integration,
+// one pair per thread, grid-stride over the list.
+template <int MODE, class R>
+__global__ void __launch_bounds__(256) k_frag_wide(CellArgs a, const int32_t *__restrict__ pair_cell,
+                                                   const int32_t *__restrict__ pair_tri) {
+  const int cnt = *a.novf;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < cnt; idx += gridDim.x * blockDim.x) {
+    const int64_t p = a.ovf[idx];
+    const int64_t k = pair_cell[p];
+    const TriData t = a.tri[pair_tri[p]];
+    Stage<R> s1, s2, s3;
+    Integrator<MODE, R> I;
+    frag_setup<MODE, R>(a, k, t, s1, s2, s3, I);
+    frag_clip<R>(a, k, a.pn[k], s1, s2, s3, [&](const SV<R> &v) { I.push(v); });
+    I.finish();
+    if (!(I.cnt >= 3 && I.positive())) continue;
+    if (MODE == 0) {
+      atomicAdd(&a.mass[k], (double)I.mass - (double)I.cmass);
+      atomicAdd(&a.intg[k], (double)I.intg - (double)I.cintg);
+    } else {
+      atomicAdd(&a.mass[k], (double)I.mass);
+      atomicAdd(&a.mom[2 * k], (double)I.mx);
+      atomicAdd(&a.mom[2 * k + 1], (double)I.my);
+    }
+    if (MODE == 3) {
+      atomicAdd(&a.mom2[3 * k], (double)I.sxx);
+      atomicAdd(&a.mom2[3 * k + 1], (double)I.syy);
+      atomicAdd(&a.mom2[3 * k + 2], (double)I.sxy);
+    }
+    if (MODE == 2) {
+      const int64_t ti = pair_tri[p];
+      for (int v = 0; v < 3; v++) atomicAdd(&a.dval[a.triv[3 * ti + v]], I.dv[v]);
+    }
+    atomicAdd(&a.counters[0], 1ull);
+  }
 }
 
 // Hessian row of every fast-path cell (include/MA/kantorovich.hpp:109-122):
@@ -1233,18 +1325,29 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
                      B.pair_cell.p, B.pair_tri.p);
   MA_HIP_TRY(hipEventRecord(e->ev[4], e->stream));
   if (np > 0) {
-    const dim3 grid(nblocks(np, 256)), block(256);
+    // list of the pairs whose fragment outgrows k_frag's staging (count at [np])
+    MA_HIP_TRY(B.ovf.ensure((size_t)np + 1));
+    MA_HIP_TRY(hipMemsetAsync(B.ovf.p + np, 0, sizeof(int32_t), e->stream));
+    a.ovf = B.ovf.p;
+    a.novf = B.ovf.p + np;
+    const dim3 grid(nblocks(np, 256)), block(256), wgrid(64);
     const int32_t *pc = B.pair_cell.p, *pt = B.pair_tri.p;
-    if (mode == 0 && fp32)
+    if (mode == 0 && fp32) {
       hipLaunchKernelGGL((k_frag<0, float>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
-    else if (mode == 0)
+      hipLaunchKernelGGL((k_frag_wide<0, float>), wgrid, block, 0, e->stream, a, pc, pt);
+    } else if (mode == 0) {
       hipLaunchKernelGGL((k_frag<0, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
-    else if (mode == 1)
+      hipLaunchKernelGGL((k_frag_wide<0, double>), wgrid, block, 0, e->stream, a, pc, pt);
+    } else if (mode == 1) {
       hipLaunchKernelGGL((k_frag<1, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
-    else if (mode == 3)
+      hipLaunchKernelGGL((k_frag_wide<1, double>), wgrid, block, 0, e->stream, a, pc, pt);
+    } else if (mode == 3) {
       hipLaunchKernelGGL((k_frag<3, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
-    else
+      hipLaunchKernelGGL((k_frag_wide<3, double>), wgrid, block, 0, e->stream, a, pc, pt);
+    } else {
       hipLaunchKernelGGL((k_frag<2, double>), grid, block, 0, e->stream, a, (int64_t)np, pc, pt);
+      hipLaunchKernelGGL((k_frag_wide<2, double>), wgrid, block, 0, e->stream, a, pc, pt);
+    }
   }
   MA_HIP_TRY(hipEventRecord(e->ev[5], e->stream));
   if (mode == 0 && fp32)

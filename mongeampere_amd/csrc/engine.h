@@ -120,6 +120,7 @@ struct EvalBuffers {
   DBuf<double> px, py, eacc, bbox;
   DBuf<float> pxf, pyf, eaccf;   // FP32 polygons and edge integrals (mixed mode)
   DBuf<int32_t> plab, pn, tcount, poff, pair_cell, pair_tri, glist;
+  DBuf<int32_t> ovf;          // pairs for k_frag_wide (+ count)
   // cells beyond the general kernel (list + count at [n]) and their long rows
   DBuf<int32_t> hlist, hrow_col;
   DBuf<double> hrow_val;
