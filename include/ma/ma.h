@@ -167,6 +167,50 @@ ma_status ma_first_moment(ma_engine *e, const double *w, double *mass, double *m
 ma_status ma_second_moment(ma_engine *e, const double *w, double *mass, double *mom,
                            double *inertia);
 
+/* Polygons of every fragment (Laguerre cell of Dirac i cut by triangle t) of
+ * positive area at weights w, in FP64; replaces the traversal
+ * MA::voronoi_triangulation_intersection
+ * (include/MA/voronoi_triangulation_intersection.hpp:315-343), which hands
+ * each fragment to a callback out(polygon, face, vertex).  Fragments come
+ * sorted by (Dirac, triangle): fragment f is Dirac cell[f] (caller index)
+ * cut by triangle tri[f] (index into the mesh's triangle list), with
+ * vertices xy[2*start[f] .. 2*start[f+1]) (x, y interleaved, absolute
+ * coordinates, counter-clockwise); start has nfrag + 1 entries.  *nfrag and
+ * *nvert always receive the sizes; if cap_frag < *nfrag or cap_vert < *nvert
+ * nothing is copied and MA_CAPACITY is returned (query with 0 and NULL
+ * arrays).  Fragments of zero area (a cell touching a triangle along an edge
+ * or at a point) are not listed. */
+ma_status ma_fragments(ma_engine *e, const double *w, int64_t cap_frag, int64_t cap_vert,
+                       int64_t *nfrag, int64_t *nvert, int32_t *cell, int32_t *tri,
+                       int64_t *start, double *xy);
+
+/* The Laguerre diagram rasterised, replacing MA::draw_laguerre_diagram
+ * (include/MA/rasterization.hpp:491-547) with a put_pixel that sums: the box
+ * [x0,x1] x [y0,y1] maps to a width x height pixel grid (pixel (i, j) =
+ * [i, i+1] x [j, j+1] in pixel units, row j counted from the box's lower
+ * edge), and every fragment adds, to each pixel it covers,
+ *   cover x (mean of the density over the fragment's vertices) x colors[cell]
+ * with cover = area of fragment cap pixel in pixel units (exact coverage, as
+ * rasterize_convex_polygon, :403-470); pixels outside the grid are dropped.
+ * colors: n x nc (caller order); img: width x height x nc, written
+ * img[(j * width + i) * nc + c] (overwritten).  device_ptrs = 1: w, colors
+ * and img are device pointers. */
+ma_status ma_draw_laguerre(ma_engine *e, const double *w, const double *colors, int nc,
+                           double x0, double y0, double x1, double y1, int width, int height,
+                           double *img, int device_ptrs);
+
+/* Mesh and density from a grey-level image, replacing MA::image_to_pl_function
+ * (include/MA/functions.hpp:82-120): pixels is width x height, row-major with
+ * the first row at the top (pixel (x, y) at pixels[y * width + x], the layout
+ * of CImg's image(x, y)); vertex (i, j) = (-1 + 2i/(width-1), -1 + 2j/(height-1)),
+ * index i * height + j, density value pixels(i, height-1-j)/255 + 1e-3; each
+ * grid square is split along its (i,j)-(i+1,j+1) diagonal (the reference's
+ * Delaunay triangulation picks a diagonal of its own on these cocircular
+ * squares), triangles 2 (i (height-1) + j) + {0, 1}.  *total (may be NULL)
+ * receives the integral of the density, the reference's return value. */
+ma_status ma_set_mesh_image(ma_engine *e, int width, int height, const double *pixels,
+                            double *total);
+
 /* Adjoint of the transport cost.  With the dual value
  *   Phi(Y, w, rho) = int rho(x) min_i (|x - y_i|^2 - w_i) dx,
  * W2^2(rho, sum_i nu_i delta_{y_i}) = max_w Phi + <nu, w>, so at the optimal

@@ -21,7 +21,7 @@ STATUS = {
     4: "MA_NOT_CONVERGED", 5: "MA_LINESEARCH_FAIL", 6: "MA_OUT_OF_MEMORY",
     7: "MA_HIP_ERROR", 8: "MA_CAPACITY", 9: "MA_NO_DATA",
 }
-MA_OK, MA_EMPTY_CELL, MA_NOT_CONVERGED, MA_LINESEARCH_FAIL = 0, 2, 4, 5
+MA_OK, MA_EMPTY_CELL, MA_NOT_CONVERGED, MA_LINESEARCH_FAIL, MA_CAPACITY = 0, 2, 4, 5, 8
 
 
 class SolveOpts(ctypes.Structure):
@@ -60,6 +60,9 @@ SIGNATURES = [
     ("ma_solve_laplacian_csr", I32, [P, I64, P, P, P, P, P, F64, I32, P, P]),
     ("ma_first_moment", I32, [P, P, P, P]),
     ("ma_second_moment", I32, [P, P, P, P, P]),
+    ("ma_fragments", I32, [P, P, I64, I64, P, P, P, P, P, P]),
+    ("ma_draw_laguerre", I32, [P, P, P, I32, F64, F64, F64, F64, I32, I32, P, I32]),
+    ("ma_set_mesh_image", I32, [P, I32, I32, P, P]),
     ("ma_adjoint", I32, [P, P, P, P]),
     ("ma_get_eval_info", I32, [P, P]),
     ("ma_cell_paths", I32, [P, I64, P, P]),

@@ -10,6 +10,13 @@ Reference entry points and their counterparts here:
   (``include/MA/optimal_transport.hpp:89-193``) -> ``ot_solve(mesh, X, masses, ...)``
 * ``MA::first_moment`` / ``MA::second_moment`` / ``MA::lloyd``
   (``include/MA/lloyd.hpp:30-144``)             -> ``first_moment`` / ``second_moment`` / ``lloyd``
+* ``MA::voronoi_triangulation_intersection(t, dt, out)``
+  (``include/MA/voronoi_triangulation_intersection.hpp:315-343``)
+                                                -> ``voronoi_triangulation_intersection``
+* ``MA::draw_laguerre_diagram`` (``include/MA/rasterization.hpp:491-547``)
+                                                -> ``draw_laguerre_diagram``
+* ``MA::image_to_pl_function`` (``include/MA/functions.hpp:82-120``)
+                                                -> ``Engine.set_mesh_image``
 
 Everything runs on the GPU through ``libmagpu.so``; this module only moves
 arrays across the C ABI.
@@ -62,6 +69,7 @@ class Engine:
             raise EngineError(st, "ma_create failed (no GPU visible?)")
         self._h = h
         self.n = 0
+        self.nv = 0
         self.mesh = None
 
     def close(self):
@@ -104,7 +112,24 @@ class Engine:
         self._check(self._L.ma_set_mesh(self._h, vxy.shape[0], _ptr(vxy), _ptr(vval),
                                         tri.shape[0], _ptr(tri)))
         self.mesh = mesh
+        self.nv = vxy.shape[0]
         return self
+
+    def set_mesh_image(self, image):
+        """``MA::image_to_pl_function`` (include/MA/functions.hpp:82-120): the
+        P1 density of a grey-level image (rows x columns, first row at the
+        top, values 0..255), built on the device: vertex (i, j) =
+        (-1 + 2i/(W-1), -1 + 2j/(H-1)), index i*H + j, value
+        image[H-1-j, i]/255 + 1e-3.  Returns the total mass."""
+        img = np.ascontiguousarray(image, np.float64)
+        if img.ndim != 2:
+            raise ValueError("image must be a 2-D array")
+        H, W = img.shape
+        total = ctypes.c_double()
+        self._check(self._L.ma_set_mesh_image(self._h, W, H, _ptr(img), ctypes.byref(total)))
+        self.mesh = None
+        self.nv = W * H
+        return total.value
 
     def set_diracs(self, X, layout="xy"):
         X = np.asarray(X, np.float64)
@@ -246,9 +271,58 @@ class Engine:
     def adjoint(self, w):
         w = np.ascontiguousarray(w, np.float64)
         dY = np.empty((self.n, 2))
-        dval = np.empty(self.mesh.vxy.shape[0])
+        dval = np.empty(self.nv)
         self._check(self._L.ma_adjoint(self._h, _ptr(w), _ptr(dY), _ptr(dval)))
         return dY, dval
+
+    # ---- fragments / rasterisation ---------------------------------------
+    def fragments(self, w):
+        """Every fragment (Laguerre cell cut by a triangle) of positive area,
+        sorted by (Dirac, triangle): ``(cell, tri, start, xy)``, fragment f
+        with vertices ``xy[start[f]:start[f+1]]`` counter-clockwise
+        (include/MA/voronoi_triangulation_intersection.hpp:315-343)."""
+        w = np.ascontiguousarray(w, np.float64)
+        nf = ctypes.c_int64()
+        nvx = ctypes.c_int64()
+        st = self._L.ma_fragments(self._h, _ptr(w), 0, 0, ctypes.byref(nf), ctypes.byref(nvx),
+                                  None, None, None, None)
+        if st not in (0, _lib.MA_CAPACITY):
+            self._check(st)
+        cell = np.empty(max(nf.value, 1), np.int32)
+        tri = np.empty(max(nf.value, 1), np.int32)
+        start = np.empty(nf.value + 1, np.int64)
+        xy = np.empty((max(nvx.value, 1), 2))
+        self._check(self._L.ma_fragments(self._h, _ptr(w), nf.value, nvx.value,
+                                         ctypes.byref(nf), ctypes.byref(nvx), _ptr(cell),
+                                         _ptr(tri), _ptr(start), _ptr(xy)))
+        return cell[:nf.value], tri[:nf.value], start, xy[:nvx.value]
+
+    def draw_laguerre_diagram(self, w, colors, box, width, height):
+        """``MA::draw_laguerre_diagram`` (include/MA/rasterization.hpp:491-547)
+        with a summing put_pixel: image (height, width, channels), row j
+        counted from the box's lower edge; each fragment adds its exact
+        pixel coverage x the mean density over its vertices x colors[cell]."""
+        w = np.ascontiguousarray(w, np.float64)
+        col = np.asarray(colors, np.float64)
+        if col.ndim == 1:
+            col = col[:, None]
+        col = np.ascontiguousarray(col)
+        if col.shape[0] != self.n:
+            raise ValueError("one colour per Dirac")
+        img = np.empty((int(height), int(width), col.shape[1]))
+        self._check(self._L.ma_draw_laguerre(self._h, _ptr(w), _ptr(col), col.shape[1],
+                                             float(box[0]), float(box[1]), float(box[2]),
+                                             float(box[3]), int(width), int(height), _ptr(img),
+                                             0))
+        return img
+
+    def draw_laguerre_device(self, w_dev, colors_dev, nc, box, width, height, img_dev):
+        """Device-pointer variant of ``draw_laguerre_diagram`` (addresses of
+        float64 tensors: weights n, colours n x nc, image height x width x nc)."""
+        self._check(self._L.ma_draw_laguerre(
+            self._h, ctypes.c_void_p(w_dev), ctypes.c_void_p(colors_dev), int(nc),
+            float(box[0]), float(box[1]), float(box[2]), float(box[3]), int(width), int(height),
+            ctypes.c_void_p(img_dev), 1))
 
 
 # ---- reference-shaped free functions ---------------------------------------
@@ -291,3 +365,27 @@ def lloyd(mesh, X, weights):
 def second_moment(mesh, X, weights):
     """``MA::second_moment``: (masses, first moments, second moments)."""
     return _engine(mesh, X).second_moment(weights)
+
+
+def voronoi_triangulation_intersection(mesh, X, weights, out=None):
+    """``MA::voronoi_triangulation_intersection``: with ``out`` given, calls
+    ``out(polygon, triangle, dirac)`` for every fragment (polygon: (k, 2)
+    array); without, returns the arrays of ``Engine.fragments``."""
+    cell, tri, start, xy = _engine(mesh, X).fragments(weights)
+    if out is None:
+        return cell, tri, start, xy
+    for f in range(cell.shape[0]):
+        out(xy[start[f]:start[f + 1]], int(tri[f]), int(cell[f]))
+    return None
+
+
+def draw_laguerre_diagram(mesh, X, weights, colors, box, width, height):
+    """``MA::draw_laguerre_diagram`` with a summing put_pixel."""
+    return _engine(mesh, X).draw_laguerre_diagram(weights, colors, box, width, height)
+
+
+def image_to_pl_function(image, engine=None):
+    """``MA::image_to_pl_function``: loads the image's P1 density into an
+    engine (a new one by default); returns ``(engine, total_mass)``."""
+    e = engine if engine is not None else Engine()
+    return e, e.set_mesh_image(image)

@@ -262,8 +262,9 @@ def image_to_pl(image):
     square is split along its (i,j)-(i+1,j+1) diagonal."""
     img = np.asarray(image, dtype=np.float64)
     m, n = img.shape  # rows = height, cols = width
-    xs = -1.0 + 2.0 * np.arange(n) / (n - 1)
-    ys = -1.0 + 2.0 * np.arange(m) / (m - 1)
+    # x0 + i dx with dx = 2 / (n - 1), rounded as functions.hpp:95-96 rounds
+    xs = -1.0 + np.arange(n) * (2.0 / (n - 1))
+    ys = -1.0 + np.arange(m) * (2.0 / (m - 1))
     X, Y = np.meshgrid(xs, ys, indexing="ij")
     vxy = np.stack([X.ravel(), Y.ravel()], axis=1)
     ii, jj = np.meshgrid(np.arange(n), np.arange(m), indexing="ij")

@@ -82,6 +82,7 @@ struct CellArgs {
   int32_t *flags;
   unsigned long long *counters;
   int force_general;  // route every cell to the general kernel (testing)
+  FragOut fo;         // fragment polygons (mode 1, when fo.count is set)
 };
 
 template <class R>
@@ -937,7 +938,7 @@ __device__ inline void frag_clip(const CellArgs &a, int64_t k, int n, Stage<R> &
 // Fragment vertices staged per thread in LDS by k_frag (FP32 / FP64).
 template <class R>
 struct FragCap {
-  static constexpr int value = sizeof(R) == sizeof(float) ? 12 : 15;
+  static constexpr int value = sizeof(R) == sizeof(float) ? 9 : 10;
 };
 
 template <int MODE, class R>
@@ -991,6 +992,18 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
     I.finish();
   }
   const bool counted = valid && !wide && I.cnt >= 3 && I.positive();
+  if (MODE == 1 && a.fo.count != nullptr) {
+    // fragment polygons from the staging (every lane takes part)
+    auto gx = [&](int i) { return ox[i * 256 + tt]; };
+    auto gy = [&](int i) { return oy[i * 256 + tt]; };
+    const double eps = frag_eps(I.xlo, I.xhi, I.ylo, I.yhi);
+    const int m = counted ? frag_distinct(nout, eps, gx, gy) : 0;
+    unsigned long long f, v;
+    frag_reserve_wave(a.fo, m, f, v);
+    if (m > 0)
+      frag_write(a.fo, f, v, m, a.perm[k], pair_tri[p], a.pos[2 * k], a.pos[2 * k + 1], nout,
+                 eps, gx, gy);
+  }
   if (counted && MODE == 2) {
     const int64_t ti = pair_tri[p];
 #pragma unroll
@@ -1079,6 +1092,34 @@ __global__ void __launch_bounds__(256) k_frag_wide(CellArgs a, const int32_t *__
     frag_clip<R>(a, k, a.pn[k], s1, s2, s3, [&](const SV<R> &v) { I.push(v); });
     I.finish();
     if (!(I.cnt >= 3 && I.positive())) continue;
+    if (MODE == 1 && a.fo.count != nullptr) {
+      // the fragment twice more: its distinct vertices counted, then written
+      const double eps = frag_eps(I.xlo, I.xhi, I.ylo, I.yhi);
+      DistinctRun d(eps);
+      Integrator<MODE, R> J;  // (I still holds this pair's sums)
+      frag_setup<MODE, R>(a, k, t, s1, s2, s3, J);
+      frag_clip<R>(a, k, a.pn[k], s1, s2, s3,
+                   [&](const SV<R> &q) { d.take((double)q.x, (double)q.y); });
+      const int m = d.count();
+      unsigned long long f, v;
+      frag_reserve(a.fo, m, f, v);
+      if (f < (unsigned long long)a.fo.cap_frag && v + m <= (unsigned long long)a.fo.cap_vert) {
+        a.fo.cell[f] = a.perm[k];
+        a.fo.tri[f] = pair_tri[p];
+        a.fo.nv[f] = m;
+        a.fo.start[f] = (int64_t)v;
+        const double ox = a.pos[2 * k], oy = a.pos[2 * k + 1];
+        DistinctRun dw(eps);
+        frag_setup<MODE, R>(a, k, t, s1, s2, s3, J);
+        frag_clip<R>(a, k, a.pn[k], s1, s2, s3, [&](const SV<R> &q) {
+          const int at = dw.cnt;
+          if (dw.take((double)q.x, (double)q.y) && at < m) {
+            a.fo.xy[2 * (v + at)] = (double)q.x + ox;
+            a.fo.xy[2 * (v + at) + 1] = (double)q.y + oy;
+          }
+        });
+      }
+    }
     if (MODE == 0) {
       atomicAdd(&a.mass[k], (double)I.mass - (double)I.cmass);
       atomicAdd(&a.intg[k], (double)I.intg - (double)I.cintg);
@@ -1267,6 +1308,8 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   a.bin_start = e->bin_start.p;
   a.triv = e->triv.p;
   a.dval = B.dval.p;
+  a.fo = e->fo;
+  if (mode != 1) a.fo.count = nullptr;
   for (int i = 0; i < 4; i++) a.dom[i] = e->dom[i];
   const int32_t zero = 0;
   PyrOff po{};

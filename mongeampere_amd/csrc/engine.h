@@ -91,6 +91,36 @@ constexpr int kBigRect = 64;
 constexpr int kHugeRoute = 32768;
 constexpr int kLongRows = 1024;  // pool of staged rows longer than kMaxRow
 
+// Fragment polygons written out by the first-moment kernels while `count` is
+// set (ma_fragments).  Each fragment of positive area reserves one record and
+// its vertices; nothing is written past the capacities, but the counts keep
+// advancing, so a pass that ran out learns the sizes it needs.
+struct FragOut {
+  unsigned long long *count = nullptr;  // [0] fragments, [1] vertices
+  int32_t *cell = nullptr;  // caller index of the Dirac
+  int32_t *tri = nullptr;   // triangle index (caller order)
+  int32_t *nv = nullptr;    // vertex count
+  int64_t *start = nullptr; // first vertex
+  double *xy = nullptr;     // absolute coordinates, interleaved, counter-clockwise
+  int64_t cap_frag = 0, cap_vert = 0;
+};
+
+// Buffers of the output entry points (fragment polygons, rasterisation).
+struct OutBuffers {
+  DBuf<unsigned long long> fcount;
+  DBuf<int32_t> fcell, ftri, fnv;
+  DBuf<int64_t> fstart;
+  DBuf<double> fxy;
+  DBuf<uint64_t> key, key2;   // radix sort by (Dirac, triangle)
+  DBuf<int32_t> idx, idx2;
+  DBuf<unsigned char> sort_tmp;
+  DBuf<int64_t> scnt, sstart; // vertex counts and offsets in sorted order
+  DBuf<double> sxy;           // sorted vertices
+  DBuf<int32_t> scell, stri;
+  DBuf<double> colors, img;   // rasterisation
+  DBuf<double> pix;           // image pixels (mesh from an image)
+};
+
 struct EvalBuffers {
   DBuf<double> w_s;          // weights in Morton order
   DBuf<double> pyr;          // max-weight pyramid (all levels)
@@ -171,6 +201,8 @@ struct ma_engine {
 
   ma::EvalBuffers eb;
   ma::SolverBuffers sb;
+  ma::OutBuffers ob;
+  ma::FragOut fo;  // set only while ma_fragments / ma_draw_laguerre run
   ma_eval_info info{};
   bool have_eval = false;
   int64_t nnz = 0;
@@ -180,6 +212,15 @@ struct ma_engine {
 };
 
 namespace ma {
+
+// Checks that mesh and Diracs are set and builds the triangle bins if needed
+// (engine.hip).
+ma_status need(ma_engine *e);
+// Completes a mesh whose vertices (e->vxy, e->vval) are on the device: triangle
+// records and bounding boxes from the device triangle list d_tri (nt x 3),
+// domain box `box` (x0, y0, x1, y1).
+ma_status finish_mesh(ma_engine *e, int64_t nv, int64_t nt, const int32_t *d_tri,
+                      const double box[4]);
 
 // Geometry (geometry.hip)
 hipError_t build_dirac_grid(ma_engine *e, const double *d_X, int layout);

@@ -425,7 +425,7 @@ static hipError_t eval_csr(ma_engine *e, CsrView &M, int32_t &gi) {
 
 using namespace ma;
 
-static ma_status need(ma_engine *e) {
+ma_status ma::need(ma_engine *e) {
   if (!e) return MA_BAD_INPUT;
   if (!e->have_mesh || !e->have_diracs) {
     e->err = "mesh and Diracs must be set first";
@@ -546,25 +546,39 @@ ma_status ma_set_mesh(ma_engine *e, int64_t nv, const double *vxy, const double 
                          y1 > e->grid.y0 + e->grid.G * e->grid.h)) {
     e->have_diracs = false;
   }
-  e->dom[0] = x0; e->dom[1] = y0; e->dom[2] = x1; e->dom[3] = y1;
   MA_HIP_TRY(e->vxy.ensure((size_t)(2 * nv)));
   MA_HIP_TRY(e->vval.ensure((size_t)nv));
-  MA_HIP_TRY(e->tri.ensure((size_t)nt));
-  MA_HIP_TRY(e->triv.ensure((size_t)(3 * nt)));
-  MA_HIP_TRY(e->tbox.ensure((size_t)nt));
-  DBuf<int32_t> dtri, bad;
+  DBuf<int32_t> dtri;
   MA_HIP_TRY(dtri.ensure((size_t)(3 * nt)));
-  MA_HIP_TRY(bad.ensure(1));
   MA_HIP_TRY(hipMemcpyAsync(e->vxy.p, vxy, sizeof(double) * 2 * nv, hipMemcpyHostToDevice, e->stream));
   MA_HIP_TRY(hipMemcpyAsync(e->vval.p, vval, sizeof(double) * nv, hipMemcpyHostToDevice, e->stream));
   MA_HIP_TRY(hipMemcpyAsync(dtri.p, tri, sizeof(int32_t) * 3 * nt, hipMemcpyHostToDevice, e->stream));
+  const double box[4] = {x0, y0, x1, y1};
+  return finish_mesh(e, nv, nt, dtri.p, box);
+}
+
+}  // extern "C"
+
+ma_status ma::finish_mesh(ma_engine *e, int64_t nv, int64_t nt, const int32_t *d_tri,
+                          const double box[4]) {
+  // a mesh vertex outside the current grid invalidates the Dirac binning
+  if (e->have_diracs && (box[0] < e->grid.x0 || box[1] < e->grid.y0 ||
+                         box[2] > e->grid.x0 + e->grid.G * e->grid.h ||
+                         box[3] > e->grid.y0 + e->grid.G * e->grid.h)) {
+    e->have_diracs = false;
+  }
+  for (int i = 0; i < 4; i++) e->dom[i] = box[i];
+  MA_HIP_TRY(e->tri.ensure((size_t)nt));
+  MA_HIP_TRY(e->triv.ensure((size_t)(3 * nt)));
+  MA_HIP_TRY(e->tbox.ensure((size_t)nt));
+  DBuf<int32_t> bad;
+  MA_HIP_TRY(bad.ensure(1));
   MA_HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int32_t), e->stream));
   hipLaunchKernelGGL(k_tri_prep, dim3((unsigned)((nt + 127) / 128)), dim3(128), 0, e->stream,
-                     e->vxy.p, e->vval.p, dtri.p, nt, e->tri.p, e->triv.p, e->tbox.p, bad.p);
+                     e->vxy.p, e->vval.p, d_tri, nt, e->tri.p, e->triv.p, e->tbox.p, bad.p);
   int32_t hb = 0;
   MA_HIP_TRY(hipMemcpyAsync(&hb, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, e->stream));
   MA_HIP_TRY(hipStreamSynchronize(e->stream));
-  dtri.release();
   bad.release();
   if (hb) {
     e->err = "degenerate (zero-area) triangle";
@@ -577,6 +591,8 @@ ma_status ma_set_mesh(ma_engine *e, int64_t nv, const double *vxy, const double 
   e->have_eval = false;
   return MA_OK;
 }
+
+extern "C" {
 
 ma_status ma_set_diracs(ma_engine *e, int64_t n, const double *X, int layout) {
   if (!e || n < 1 || !X || (layout != MA_XY_INTERLEAVED && layout != MA_XY_COLMAJOR))

@@ -124,4 +124,109 @@ __device__ inline void for_each_triangle(const Grid &g, const int32_t *__restric
     }
 }
 
+// Fragment output (FragOut): one record per fragment of positive area.
+// frag_reserve: plain atomics, for kernels whose lanes are not converged.
+__device__ inline void frag_reserve(const FragOut &o, int m, unsigned long long &f,
+                                    unsigned long long &v) {
+  f = atomicAdd(&o.count[0], 1ull);
+  v = atomicAdd(&o.count[1], (unsigned long long)m);
+}
+// frag_reserve_wave: every lane of the wave calls it (m = 0: no fragment);
+// one atomic per counter per wave, slots in lane order.
+__device__ inline void frag_reserve_wave(const FragOut &o, int m, unsigned long long &f,
+                                         unsigned long long &v) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int one = m > 0 ? 1 : 0;
+  int cf = one, cv = m;  // inclusive prefix sums over the wave
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int tf = __shfl_up(cf, d, 64);
+    const int tv = __shfl_up(cv, d, 64);
+    if (lane >= d) {
+      cf += tf;
+      cv += tv;
+    }
+  }
+  unsigned long long bf = 0, bv = 0;
+  if (lane == 63) {
+    if (cf > 0) bf = atomicAdd(&o.count[0], (unsigned long long)cf);
+    if (cv > 0) bv = atomicAdd(&o.count[1], (unsigned long long)cv);
+  }
+  bf = __shfl(bf, 63, 64);
+  bv = __shfl(bv, 63, 64);
+  f = bf + (unsigned long long)(cf - one);
+  v = bv + (unsigned long long)(cv - m);
+}
+// Repeated vertices are left out of the output: a vertex within eps of the
+// one kept before it, and a last vertex within eps of the first (zero-length
+// edges, e.g. where a cell edge and a triangle edge both lie on the domain
+// boundary: the two copies of the corner are rounded separately).  The
+// polygons of the reference's traversal carry no such repeats, and a repeat
+// would bias anything averaged over the vertices.  eps: 1e-12 of the
+// fragment's extent (frag_eps).
+struct DistinctRun {
+  int cnt = 0;
+  double eps = 0.0;
+  double fx = 0.0, fy = 0.0, lx = 0.0, ly = 0.0;  // first and last kept vertex
+  __device__ explicit DistinctRun(double e) : eps(e) {}
+  __device__ bool same(double x, double y, double u, double v) const {
+    return fabs(x - u) <= eps && fabs(y - v) <= eps;
+  }
+  // true if (x, y) is kept
+  __device__ bool take(double x, double y) {
+    if (cnt > 0 && same(x, y, lx, ly)) return false;
+    if (cnt == 0) {
+      fx = x;
+      fy = y;
+    }
+    lx = x;
+    ly = y;
+    cnt++;
+    return true;
+  }
+  __device__ int count() const { return (cnt > 1 && same(lx, ly, fx, fy)) ? cnt - 1 : cnt; }
+};
+__device__ inline double frag_eps(double xlo, double xhi, double ylo, double yhi) {
+  return 1e-12 * fmax(xhi - xlo, yhi - ylo);
+}
+template <class LX, class LY>
+__device__ inline double frag_eps_of(int n, LX lx, LY ly) {
+  double xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY;
+  for (int i = 0; i < n; i++) {
+    const double x = (double)lx(i), y = (double)ly(i);
+    xlo = fmin(xlo, x);
+    xhi = fmax(xhi, x);
+    ylo = fmin(ylo, y);
+    yhi = fmax(yhi, y);
+  }
+  return frag_eps(xlo, xhi, ylo, yhi);
+}
+template <class LX, class LY>
+__device__ inline int frag_distinct(int n, double eps, LX lx, LY ly) {
+  DistinctRun d(eps);
+  for (int i = 0; i < n; i++) d.take((double)lx(i), (double)ly(i));
+  return d.count();
+}
+// Writes the record at slots (f, v) if both fit: the m distinct vertices
+// (frag_distinct) of the n vertices (lx(i) + ox, ly(i) + oy).
+template <class LX, class LY>
+__device__ inline void frag_write(const FragOut &o, unsigned long long f, unsigned long long v,
+                                  int m, int32_t cell, int32_t tri, double ox, double oy, int n,
+                                  double eps, LX lx, LY ly) {
+  if (f >= (unsigned long long)o.cap_frag || v + m > (unsigned long long)o.cap_vert) return;
+  o.cell[f] = cell;
+  o.tri[f] = tri;
+  o.nv[f] = m;
+  o.start[f] = (int64_t)v;
+  DistinctRun d(eps);
+  for (int i = 0; i < n; i++) {
+    const double x = (double)lx(i), y = (double)ly(i);
+    const int at = d.cnt;
+    if (d.take(x, y) && at < m) {
+      o.xy[2 * (v + at)] = x + ox;
+      o.xy[2 * (v + at) + 1] = y + oy;
+    }
+  }
+}
+
 }  // namespace ma

@@ -1079,7 +1079,7 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
     int32_t *row_col,
     double *row_val, int cap_row, int32_t *hrow_col, double *hrow_val,
     const int32_t *__restrict__ triv, double *dval, int32_t *flags,
-    unsigned long long *counters) {
+    unsigned long long *counters, FragOut fo) {
   __shared__ double sx[2][kHugePoly], sy[2][kHugePoly];
   __shared__ int sl[2][kHugePoly];
   __shared__ double sacc[kHugePoly];  // clip values while building, edge integrals after
@@ -1364,6 +1364,15 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
           if (!(0.5 * ar2 > 1e-13 * ((xhi - xlo) + (yhi - ylo)) * cm)) continue;
         }
         nfrag++;
+        if (MODE == 1 && fo.count != nullptr) {
+          auto gx = [&](int i) { return fx[i]; };
+          auto gy = [&](int i) { return fy[i]; };
+          const double eps = frag_eps_of(m, gx, gy);
+          const int md = frag_distinct(m, eps, gx, gy);
+          unsigned long long f, v;
+          frag_reserve(fo, md, f, v);
+          frag_write(fo, f, v, md, perm[k], tbin_items[q], yx, yy, m, eps, gx, gy);
+        }
         double dv[3] = {0.0, 0.0, 0.0};
         frag_integrals<MODE>(fx, fy, m, t, yx, yy, wk, S, dv);
         if (MODE == 2) {
@@ -1657,9 +1666,11 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   if (cs != MA_OK) return cs;
   int32_t nhuge = 0;
   const int32_t *hl = nullptr;  // list of the cells for k_cell_huge
-  if (ngen > 0 && ngen <= kHugeRoute && e->force_general != 1) {
+  if (ngen > 0 && ((ngen <= kHugeRoute && e->force_general != 1) || e->fo.count != nullptr)) {
     // few cells: one workgroup each beats one thread each (whose candidate
-    // loops run at memory latency)
+    // loops run at memory latency).  Fragment output takes this route for
+    // all of them: the general kernel can hand a cell on to the workgroup
+    // kernel after it has met some of its fragments.
     nhuge = ngen;
     hl = B.glist.p;
   } else if (ngen > 0) {
@@ -1692,7 +1703,7 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
                        e->tbin_start.p, e->tbin_items.p, e->tri.p, B.mass_s.p, B.intg_s.p,
                        B.fval_part.p, B.mom_s.p, B.mom2_s.p, B.row_cnt.p, B.row_col.p, B.row_val.p,
                        e->cap_row, B.hrow_col.p, B.hrow_val.p, e->triv.p, B.dval.p, B.flags.p,
-                       B.counters.p);
+                       B.counters.p, mode == 1 ? e->fo : FragOut{});
   }
   MA_HIP_TRY(hipEventRecord(e->ev[6], e->stream));
   if (mode == 0) {

@@ -1,0 +1,491 @@
+// Output entry points around the hot path: the polygons of all fragments
+// (the traversal's callback interface), the Laguerre diagram rasterised into
+// an image, and the P1 density mesh of a grey-level image.
+//
+// Reference:
+//   fragment polygons     include/MA/voronoi_triangulation_intersection.hpp:315-343
+//   rasterisation         include/MA/rasterization.hpp:403-547
+//                         (rasterize_convex_polygon, rasterize_polygon, draw_laguerre_diagram)
+//   image -> P1 density   include/MA/functions.hpp:82-120 (image_to_pl_function)
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "geom_common.h"
+
+namespace ma {
+
+// ---------------------------------------------------------------------------
+// Fragment polygons
+// ---------------------------------------------------------------------------
+
+__global__ void k_frag_keys(int64_t nf, const int32_t *cell, const int32_t *tri, uint64_t *key,
+                            int32_t *idx) {
+  const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  key[f] = ((uint64_t)(uint32_t)cell[f] << 32) | (uint32_t)tri[f];
+  idx[f] = (int32_t)f;
+}
+
+__global__ void k_frag_sorted_counts(int64_t nf, const int32_t *idx, const int32_t *nv,
+                                     int64_t *cnt) {
+  const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f < nf) cnt[f] = nv[idx[f]];
+  if (f == nf) cnt[f] = 0;
+}
+
+// Fragment f of the sorted order: its record and vertices from the emission
+// slots.
+__global__ void k_frag_gather(int64_t nf, const int32_t *idx, const uint64_t *key,
+                              const int64_t *src_start, const int32_t *nv, const double *src_xy,
+                              const int64_t *dst_start, int32_t *cell, int32_t *tri,
+                              double *dst_xy) {
+  const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int32_t s = idx[f];
+  cell[f] = (int32_t)(key[f] >> 32);
+  tri[f] = (int32_t)(key[f] & 0xffffffffu);
+  const int64_t a = src_start[s], b = dst_start[f];
+  const int m = nv[s];
+  for (int i = 0; i < 2 * m; i++) dst_xy[2 * b + i] = src_xy[2 * a + i];
+}
+
+// One first-moment evaluation with fragment output at the Morton-ordered
+// weights in eb.w_s; the buffers grow (and the pass is repeated once) when
+// the previous capacity was too small.  On return ob.f* hold *nf records
+// (unsorted) and *nvert vertices.
+static ma_status emit_fragments(ma_engine *e, int64_t *nf_out, int64_t *nv_out) {
+  OutBuffers &O = e->ob;
+  MA_HIP_TRY(O.fcount.ensure(2));
+  int64_t cf = std::max<int64_t>((int64_t)O.fcell.n, 4 * (e->n + e->nt));
+  int64_t cv = std::max<int64_t>((int64_t)O.fxy.n / 2, 6 * cf);
+  for (int pass = 0; pass < 2; pass++) {
+    MA_HIP_TRY(O.fcell.ensure((size_t)cf));
+    MA_HIP_TRY(O.ftri.ensure((size_t)cf));
+    MA_HIP_TRY(O.fnv.ensure((size_t)cf));
+    MA_HIP_TRY(O.fstart.ensure((size_t)cf + 1));
+    MA_HIP_TRY(O.fxy.ensure((size_t)(2 * cv)));
+    MA_HIP_TRY(hipMemsetAsync(O.fcount.p, 0, 2 * sizeof(unsigned long long), e->stream));
+    FragOut fo;
+    fo.count = O.fcount.p;
+    fo.cell = O.fcell.p;
+    fo.tri = O.ftri.p;
+    fo.nv = O.fnv.p;
+    fo.start = O.fstart.p;
+    fo.xy = O.fxy.p;
+    fo.cap_frag = cf;
+    fo.cap_vert = cv;
+    e->fo = fo;
+    ma_status s = evaluate(e, 1, nullptr);
+    e->fo = FragOut{};
+    if (s != MA_OK) return s;
+    unsigned long long c[2];
+    MA_HIP_TRY(hipMemcpyAsync(c, O.fcount.p, sizeof c, hipMemcpyDeviceToHost, e->stream));
+    MA_HIP_TRY(hipStreamSynchronize(e->stream));
+    if ((int64_t)c[0] <= cf && (int64_t)c[1] <= cv) {
+      *nf_out = (int64_t)c[0];
+      *nv_out = (int64_t)c[1];
+      return MA_OK;
+    }
+    cf = std::max<int64_t>(cf, (int64_t)c[0]);
+    cv = std::max<int64_t>(cv, (int64_t)c[1]);
+  }
+  e->err = "fragment output did not fit after growing its buffers";
+  return MA_HIP_ERROR;
+}
+
+// Sorts the emitted fragments by (Dirac, triangle) into ob.scell / stri /
+// sstart / sxy.
+static ma_status sort_fragments(ma_engine *e, int64_t nf, int64_t nvert) {
+  OutBuffers &O = e->ob;
+  const size_t n1 = (size_t)std::max<int64_t>(nf, 1);
+  MA_HIP_TRY(O.key.ensure(n1));
+  MA_HIP_TRY(O.key2.ensure(n1));
+  MA_HIP_TRY(O.idx.ensure(n1));
+  MA_HIP_TRY(O.idx2.ensure(n1));
+  MA_HIP_TRY(O.scell.ensure(n1));
+  MA_HIP_TRY(O.stri.ensure(n1));
+  MA_HIP_TRY(O.sstart.ensure((size_t)nf + 1));
+  MA_HIP_TRY(O.sxy.ensure((size_t)std::max<int64_t>(2 * nvert, 1)));
+  if (nf == 0) {
+    MA_HIP_TRY(hipMemsetAsync(O.sstart.p, 0, sizeof(int64_t), e->stream));
+    return MA_OK;
+  }
+  hipLaunchKernelGGL(k_frag_keys, dim3(nblocks(nf)), dim3(kBlock), 0, e->stream, nf, O.fcell.p,
+                     O.ftri.p, O.key.p, O.idx.p);
+  // key bits: Dirac index above bit 32, triangle index below
+  int hi = 33;
+  while (hi < 64 && ((int64_t)1 << (hi - 32)) <= e->n) hi++;
+  size_t tmp_bytes = 0;
+  MA_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, O.key.p, O.key2.p, O.idx.p,
+                                                O.idx2.p, (int)nf, 0, hi, e->stream));
+  MA_HIP_TRY(O.scnt.ensure((size_t)nf + 1));
+  size_t scan_bytes = 0;
+  MA_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, O.scnt.p, O.sstart.p,
+                                              (int)(nf + 1), e->stream));
+  MA_HIP_TRY(O.sort_tmp.ensure(std::max(tmp_bytes, scan_bytes)));
+  MA_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(O.sort_tmp.p, tmp_bytes, O.key.p, O.key2.p,
+                                                O.idx.p, O.idx2.p, (int)nf, 0, hi, e->stream));
+  hipLaunchKernelGGL(k_frag_sorted_counts, dim3(nblocks(nf + 1)), dim3(kBlock), 0, e->stream, nf,
+                     O.idx2.p, O.fnv.p, O.scnt.p);
+  MA_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(O.sort_tmp.p, scan_bytes, O.scnt.p, O.sstart.p,
+                                              (int)(nf + 1), e->stream));
+  hipLaunchKernelGGL(k_frag_gather, dim3(nblocks(nf)), dim3(kBlock), 0, e->stream, nf, O.idx2.p,
+                     O.key2.p, O.fstart.p, O.fnv.p, O.fxy.p, O.sstart.p, O.scell.p, O.stri.p,
+                     O.sxy.p);
+  MA_HIP_TRY(hipGetLastError());
+  return MA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Rasterisation (draw_laguerre_diagram with a summing put_pixel)
+// ---------------------------------------------------------------------------
+
+// Integral over u in [i, i+1] of clamp(v(u) - j, 0, 1) along the edge
+// (ua, va) -> (ub, vb), signed so that the sum over the edges of a
+// counter-clockwise polygon is the area of polygon cap pixel [i,i+1]x[j,j+1]:
+// for a convex polygon the vertical section at u is [lo(u), hi(u)], its part
+// in the pixel row has length c(hi) - c(lo) with c(v) = clamp(v - j, 0, 1),
+// and the upper boundary runs right to left.  The u-range is cut where v
+// crosses j and j + 1, so that on each piece the clamp is 0, 1 or linear
+// (exact at the piece's midpoint).
+__device__ inline double cover_edge(double ua, double va, double ub, double vb, double i,
+                                    double j) {
+  if (ua == ub) return 0.0;
+  double sign = -1.0;
+  if (ub < ua) {
+    double t = ua; ua = ub; ub = t;
+    t = va; va = vb; vb = t;
+    sign = 1.0;
+  }
+  const double uL = fmax(ua, i), uR = fmin(ub, i + 1.0);
+  if (!(uL < uR)) return 0.0;
+  const double slope = (vb - va) / (ub - ua);
+  const double vL = va + (uL - ua) * slope - j;
+  double p1 = uL, p2 = uR;
+  if (slope != 0.0) {
+    p1 = fmin(fmax(uL - vL / slope, uL), uR);          // v = 0
+    p2 = fmin(fmax(uL + (1.0 - vL) / slope, uL), uR);  // v = 1
+    if (p2 < p1) {
+      const double t = p1; p1 = p2; p2 = t;
+    }
+  }
+  const double pts[4] = {uL, p1, p2, uR};
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; q++) {
+    const double a = pts[q], b = pts[q + 1];
+    if (!(b > a)) continue;
+    const double vm = vL + (0.5 * (a + b) - uL) * slope;
+    s += (b - a) * fmin(fmax(vm, 0.0), 1.0);
+  }
+  return sign * s;
+}
+
+struct RasterArgs {
+  const int32_t *cell, *tri, *nv;
+  const int64_t *start;
+  const double *xy;
+  const TriData *T;
+  const double *colors;
+  int nc;
+  double x0, y0, sx, sy;  // image coordinates: (sx (x - x0), sy (y - y0))
+  int W, H;
+  double *img;
+  int32_t *big;    // fragments left to k_raster_big (count at big_n)
+  int32_t *big_n;
+};
+
+constexpr int kRasterSmall = 16;  // pixels a thread covers by itself
+constexpr int kRasterMaxV = 128;  // fragment vertices staged by k_raster_big
+
+// Pixel range of fragment f and its weight: the mean of the density over its
+// vertices (rasterization.hpp:528-534).
+__device__ inline void raster_setup(const RasterArgs &r, int64_t f, int &i0, int &i1, int &j0,
+                                    int &j1, double &ff) {
+  const int m = r.nv[f];
+  const double *p = r.xy + 2 * r.start[f];
+  const TriData t = r.T[r.tri[f]];
+  double ulo = INFINITY, uhi = -INFINITY, vlo = INFINITY, vhi = -INFINITY, acc = 0.0;
+  for (int q = 0; q < m; q++) {
+    const double x = p[2 * q], y = p[2 * q + 1];
+    acc += t.a * x + t.b * y + t.c;
+    const double u = r.sx * (x - r.x0), v = r.sy * (y - r.y0);
+    ulo = fmin(ulo, u);
+    uhi = fmax(uhi, u);
+    vlo = fmin(vlo, v);
+    vhi = fmax(vhi, v);
+  }
+  ff = acc / m;
+  i0 = max((int)floor(fmax(ulo, -1.0)), 0);
+  i1 = min((int)floor(fmin(uhi, (double)r.W)), r.W - 1);
+  j0 = max((int)floor(fmax(vlo, -1.0)), 0);
+  j1 = min((int)floor(fmin(vhi, (double)r.H)), r.H - 1);
+}
+
+// uv(q, u, v): image coordinates of vertex q of fragment f.
+template <class UV>
+__device__ inline void raster_pixel(const RasterArgs &r, int64_t f, UV uv, int m, int i, int j,
+                                    double ff) {
+  double cov = 0.0, u0, v0, ua, va;
+  uv(0, u0, v0);
+  ua = u0;
+  va = v0;
+  for (int q = 0; q < m; q++) {
+    double ub = u0, vb = v0;
+    if (q + 1 < m) uv(q + 1, ub, vb);
+    cov += cover_edge(ua, va, ub, vb, (double)i, (double)j);
+    ua = ub;
+    va = vb;
+  }
+  if (!(cov > 0.0)) return;
+  const double s = cov * ff;
+  const double *col = r.colors + (int64_t)r.cell[f] * r.nc;
+  double *dst = r.img + ((int64_t)j * r.W + i) * r.nc;
+  for (int c = 0; c < r.nc; c++) atomicAdd(&dst[c], s * col[c]);
+}
+
+// One thread per fragment covering at most kRasterSmall pixels; larger ones
+// are listed for k_raster_big.
+__global__ void __launch_bounds__(256) k_raster_small(RasterArgs r, int64_t nf) {
+  const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int i0, i1, j0, j1;
+  double ff;
+  raster_setup(r, f, i0, i1, j0, j1, ff);
+  if (i1 < i0 || j1 < j0) return;
+  const int m = r.nv[f];
+  if ((int64_t)(i1 - i0 + 1) * (j1 - j0 + 1) > kRasterSmall || m > 32) {
+    r.big[atomicAdd(r.big_n, 1)] = (int32_t)f;
+    return;
+  }
+  double u[32], v[32];
+  const double *p = r.xy + 2 * r.start[f];
+  for (int q = 0; q < m; q++) {
+    u[q] = r.sx * (p[2 * q] - r.x0);
+    v[q] = r.sy * (p[2 * q + 1] - r.y0);
+  }
+  auto uv = [&](int q, double &uq, double &vq) {
+    uq = u[q];
+    vq = v[q];
+  };
+  for (int j = j0; j <= j1; j++)
+    for (int i = i0; i <= i1; i++) raster_pixel(r, f, uv, m, i, j, ff);
+}
+
+// One workgroup per listed fragment, its threads striding over the pixels.
+__global__ void __launch_bounds__(256) k_raster_big(RasterArgs r) {
+  __shared__ double u[kRasterMaxV], v[kRasterMaxV];
+  const int nb = *r.big_n;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int64_t f = r.big[b];
+    int i0, i1, j0, j1;
+    double ff;
+    raster_setup(r, f, i0, i1, j0, j1, ff);
+    const int m = r.nv[f];
+    const double *p = r.xy + 2 * r.start[f];
+    __syncthreads();  // the previous fragment's vertices are no longer read
+    for (int q = threadIdx.x; q < min(m, kRasterMaxV); q += blockDim.x) {
+      u[q] = r.sx * (p[2 * q] - r.x0);
+      v[q] = r.sy * (p[2 * q + 1] - r.y0);
+    }
+    __syncthreads();
+    const int w = i1 - i0 + 1;
+    const int64_t npix = (i1 < i0 || j1 < j0) ? 0 : (int64_t)w * (j1 - j0 + 1);
+    if (m <= kRasterMaxV) {
+      auto uv = [&](int q, double &uq, double &vq) {
+        uq = u[q];
+        vq = v[q];
+      };
+      for (int64_t q = threadIdx.x; q < npix; q += blockDim.x)
+        raster_pixel(r, f, uv, m, i0 + (int)(q % w), j0 + (int)(q / w), ff);
+    } else {
+      // a fragment longer than the staging: vertices from global memory
+      auto uv = [&](int q, double &uq, double &vq) {
+        uq = r.sx * (p[2 * q] - r.x0);
+        vq = r.sy * (p[2 * q + 1] - r.y0);
+      };
+      for (int64_t q = threadIdx.x; q < npix; q += blockDim.x)
+        raster_pixel(r, f, uv, m, i0 + (int)(q % w), j0 + (int)(q / w), ff);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// P1 density of a grey-level image
+// ---------------------------------------------------------------------------
+
+// Vertex (i, j) = (-1 + 2i/(W-1), -1 + 2j/(H-1)), index i*H + j, value
+// pixel(i, H-1-j)/255 + 1e-3 (the image's first row is its top row); square
+// (i, j) -> triangles (a, b, c), (a, c, d) with a = (i,j), b = (i+1,j),
+// c = (i+1,j+1), d = (i,j+1).
+__global__ void k_image_mesh(const double *pix, int W, int H, double *vxy, double *vval,
+                             int32_t *tri) {
+  const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (q >= (int64_t)W * H) return;
+  const int i = (int)(q / H), j = (int)(q % H);
+  vxy[2 * q] = -1.0 + i * (2.0 / (W - 1));
+  vxy[2 * q + 1] = -1.0 + j * (2.0 / (H - 1));
+  vval[q] = pix[(int64_t)(H - 1 - j) * W + i] / 255.0 + 1e-3;
+  if (i < W - 1 && j < H - 1) {
+    const int32_t a = (int32_t)q, b = a + H, c = a + H + 1, d = a + 1;
+    int32_t *t = tri + 6 * ((int64_t)i * (H - 1) + j);
+    t[0] = a; t[1] = b; t[2] = c;
+    t[3] = a; t[4] = c; t[5] = d;
+  }
+}
+
+}  // namespace ma
+
+using namespace ma;
+
+extern "C" {
+
+ma_status ma_fragments(ma_engine *e, const double *w, int64_t cap_frag, int64_t cap_vert,
+                       int64_t *nfrag, int64_t *nvert, int32_t *cell, int32_t *tri,
+                       int64_t *start, double *xy) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!w || !nfrag || !nvert || cap_frag < 0 || cap_vert < 0) return MA_BAD_INPUT;
+  hipSetDevice(e->device);
+  MA_HIP_TRY(e->eb.w_s.ensure((size_t)e->n));
+  MA_HIP_TRY(to_morton(e, w, e->eb.w_s.p, false));
+  int64_t nf = 0, nvx = 0;
+  s = emit_fragments(e, &nf, &nvx);
+  if (s != MA_OK) return s;
+  *nfrag = nf;
+  *nvert = nvx;
+  if (nf > cap_frag || nvx > cap_vert) {
+    e->err = "fragment arrays too small (sizes returned in nfrag / nvert)";
+    return MA_CAPACITY;
+  }
+  if (nf > 0 && (!cell || !tri || !start || !xy)) return MA_BAD_INPUT;
+  if (nf > INT32_MAX) {
+    e->err = "more than 2^31 fragments";
+    return MA_CAPACITY;
+  }
+  s = sort_fragments(e, nf, nvx);
+  if (s != MA_OK) return s;
+  OutBuffers &O = e->ob;
+  if (start)
+    MA_HIP_TRY(hipMemcpyAsync(start, O.sstart.p, sizeof(int64_t) * (nf + 1),
+                              hipMemcpyDeviceToHost, e->stream));
+  if (nf > 0) {
+    MA_HIP_TRY(hipMemcpyAsync(cell, O.scell.p, sizeof(int32_t) * nf, hipMemcpyDeviceToHost,
+                              e->stream));
+    MA_HIP_TRY(hipMemcpyAsync(tri, O.stri.p, sizeof(int32_t) * nf, hipMemcpyDeviceToHost,
+                              e->stream));
+    MA_HIP_TRY(hipMemcpyAsync(xy, O.sxy.p, sizeof(double) * 2 * nvx, hipMemcpyDeviceToHost,
+                              e->stream));
+  }
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  return MA_OK;
+}
+
+ma_status ma_draw_laguerre(ma_engine *e, const double *w, const double *colors, int nc,
+                           double x0, double y0, double x1, double y1, int width, int height,
+                           double *img, int device_ptrs) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!w || !colors || !img || nc < 1 || width < 1 || height < 1 || !(x1 > x0) || !(y1 > y0))
+    return MA_BAD_INPUT;
+  hipSetDevice(e->device);
+  OutBuffers &O = e->ob;
+  const int64_t npx = (int64_t)width * height * nc;
+  MA_HIP_TRY(e->eb.w_s.ensure((size_t)e->n));
+  MA_HIP_TRY(to_morton(e, w, e->eb.w_s.p, device_ptrs != 0));
+  const double *dcol = colors;
+  double *dimg = img;
+  if (!device_ptrs) {
+    MA_HIP_TRY(O.colors.ensure((size_t)(e->n * nc)));
+    MA_HIP_TRY(hipMemcpyAsync(O.colors.p, colors, sizeof(double) * e->n * nc,
+                              hipMemcpyHostToDevice, e->stream));
+    MA_HIP_TRY(O.img.ensure((size_t)npx));
+    dcol = O.colors.p;
+    dimg = O.img.p;
+  }
+  int64_t nf = 0, nvx = 0;
+  s = emit_fragments(e, &nf, &nvx);
+  if (s != MA_OK) return s;
+  MA_HIP_TRY(hipMemsetAsync(dimg, 0, sizeof(double) * npx, e->stream));
+  if (nf > 0) {
+    MA_HIP_TRY(O.idx.ensure((size_t)nf + 1));  // list for k_raster_big, count at [nf]
+    MA_HIP_TRY(hipMemsetAsync(O.idx.p + nf, 0, sizeof(int32_t), e->stream));
+    RasterArgs r;
+    r.cell = O.fcell.p;
+    r.tri = O.ftri.p;
+    r.nv = O.fnv.p;
+    r.start = O.fstart.p;
+    r.xy = O.fxy.p;
+    r.T = e->tri.p;
+    r.colors = dcol;
+    r.nc = nc;
+    r.x0 = x0;
+    r.y0 = y0;
+    r.sx = width / (x1 - x0);
+    r.sy = height / (y1 - y0);
+    r.W = width;
+    r.H = height;
+    r.img = dimg;
+    r.big = O.idx.p;
+    r.big_n = O.idx.p + nf;
+    hipLaunchKernelGGL(k_raster_small, dim3(nblocks(nf, 256)), dim3(256), 0, e->stream, r, nf);
+    hipLaunchKernelGGL(k_raster_big, dim3(1024), dim3(256), 0, e->stream, r);
+  }
+  if (!device_ptrs)
+    MA_HIP_TRY(hipMemcpyAsync(img, O.img.p, sizeof(double) * npx, hipMemcpyDeviceToHost,
+                              e->stream));
+  MA_HIP_TRY(hipGetLastError());
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  return MA_OK;
+}
+
+ma_status ma_set_mesh_image(ma_engine *e, int width, int height, const double *pixels,
+                            double *total) {
+  if (!e || width < 2 || height < 2 || !pixels) return MA_BAD_INPUT;
+  const int64_t nv = (int64_t)width * height;
+  const int64_t nt = 2 * (int64_t)(width - 1) * (height - 1);
+  if (nt > (int64_t)INT32_MAX / 4) return MA_BAD_INPUT;
+  for (int64_t q = 0; q < nv; q++)
+    if (!std::isfinite(pixels[q])) {
+      e->err = "non-finite pixel";
+      return MA_BAD_INPUT;
+    }
+  hipSetDevice(e->device);
+  MA_HIP_TRY(e->ob.pix.ensure((size_t)nv));
+  MA_HIP_TRY(e->vxy.ensure((size_t)(2 * nv)));
+  MA_HIP_TRY(e->vval.ensure((size_t)nv));
+  DBuf<int32_t> dtri;
+  MA_HIP_TRY(dtri.ensure((size_t)(3 * nt)));
+  MA_HIP_TRY(hipMemcpyAsync(e->ob.pix.p, pixels, sizeof(double) * nv, hipMemcpyHostToDevice,
+                            e->stream));
+  hipLaunchKernelGGL(k_image_mesh, dim3(nblocks(nv)), dim3(kBlock), 0, e->stream, e->ob.pix.p,
+                     width, height, e->vxy.p, e->vval.p, dtri.p);
+  const double box[4] = {-1.0, -1.0, 1.0, 1.0};
+  ma_status s = finish_mesh(e, nv, nt, dtri.p, box);
+  if (s != MA_OK) return s;
+  if (total) {
+    // integral of the P1 function (sum of area x mean vertex value), the
+    // return value of image_to_pl_function (functions.hpp:106-118)
+    const double A = 0.5 * (2.0 / (width - 1)) * (2.0 / (height - 1));
+    double t = 0.0;
+    for (int i = 0; i + 1 < width; i++) {
+      double col = 0.0;
+      for (int j = 0; j + 1 < height; j++) {
+        auto val = [&](int ii, int jj) {
+          return pixels[(int64_t)(height - 1 - jj) * width + ii] / 255.0 + 1e-3;
+        };
+        const double a = val(i, j), b = val(i + 1, j), c = val(i + 1, j + 1), d = val(i, j + 1);
+        col += (a + b + c) + (a + c + d);
+      }
+      t += col;
+    }
+    *total = t * A / 3.0;
+  }
+  return MA_OK;
+}
+
+}  // extern "C"

@@ -758,7 +758,8 @@ static void vertex_point(const tictx *c, const pedge *e1, const pedge *e2, doubl
 
 /* ====================================================================== */
 /* Driver: kantorovich (kantorovich.hpp:35-141) / first_moment (mode 1,   */
-/* lloyd.hpp:30-69) / second_moment (mode 2, lloyd.hpp:71-123).           */
+/* lloyd.hpp:30-69) / second_moment (mode 2, lloyd.hpp:71-123) / fragment */
+/* polygons (mode 3, voronoi_triangulation_intersection.hpp:315-343).     */
 /* ====================================================================== */
 struct orc_result {
   int status;
@@ -769,7 +770,49 @@ struct orc_result {
   int32_t *ti, *tj;
   double *tv;
   int64_t stats[6];
+  /* mode 3: fragment f is Dirac fc[f] cut by triangle ft[f]; its vertices
+     are fxy[2 * fs[f] ...] up to fs[f + 1] */
+  int64_t nf, capf, nvx, capv;
+  int32_t *fc, *ft;
+  int64_t *fs;
+  double *fxy;
 };
+
+static int push_frag(orc_result *r, int32_t v, int32_t f, int m, const double *px,
+                     const double *py) {
+  if (r->nf + 1 >= r->capf) {
+    int64_t nc = r->capf ? 2 * r->capf : 1024;
+    int32_t *a = realloc(r->fc, (size_t)nc * sizeof(int32_t));
+    if (!a) return -1;
+    r->fc = a;
+    int32_t *b = realloc(r->ft, (size_t)nc * sizeof(int32_t));
+    if (!b) return -1;
+    r->ft = b;
+    int64_t *c = realloc(r->fs, (size_t)(nc + 1) * sizeof(int64_t));
+    if (!c) return -1;
+    r->fs = c;
+    r->capf = nc;
+  }
+  if (r->nvx + m > r->capv) {
+    int64_t nc = r->capv ? 2 * r->capv : 4096;
+    while (nc < r->nvx + m) nc *= 2;
+    double *d = realloc(r->fxy, (size_t)(2 * nc) * sizeof(double));
+    if (!d) return -1;
+    r->fxy = d;
+    r->capv = nc;
+  }
+  r->fc[r->nf] = v;
+  r->ft[r->nf] = f;
+  r->fs[r->nf] = r->nvx;
+  for (int k = 0; k < m; k++) {
+    r->fxy[2 * (r->nvx + k)] = px[k];
+    r->fxy[2 * (r->nvx + k) + 1] = py[k];
+  }
+  r->nvx += m;
+  r->nf++;
+  r->fs[r->nf] = r->nvx;
+  return 0;
+}
 
 static int push_trip(orc_result *r, int32_t i, int32_t j, double v) {
   if (r->nt == r->cap) {
@@ -952,7 +995,29 @@ orc_result *orc_run(int mode, int64_t nv, const double *vxy, const double *vval,
             int kk = (k == 0) ? m - 1 : k - 1;
             vertex_point(&tc, &cp[k], &cp[kk], &px[k], &py[k]);
           }
-          if (mode == 0) {
+          if (mode == 3) {
+            /* the polygon handed to the traversal's callback
+               (voronoi_triangulation_intersection.hpp:329-338); fragments of
+               zero area (a cell touching the triangle along an edge or at a
+               point) are left out, with the same scale-aware threshold on
+               coordinates local to the Dirac as the integrals use */
+            double ar2 = 0.0, xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY;
+            double cm = 0.0;
+            for (int k = 0; k < m; k++) {
+              int k1 = (k + 1) % m;
+              double ax = px[k] - yvx, ay = py[k] - yvy;
+              double bx = px[k1] - yvx, by = py[k1] - yvy;
+              ar2 += ax * by - bx * ay;
+              xlo = fmin(xlo, ax); xhi = fmax(xhi, ax);
+              ylo = fmin(ylo, ay); yhi = fmax(yhi, ay);
+              cm = fmax(cm, fmax(fabs(ax), fabs(ay)));
+            }
+            if (!(0.5 * ar2 > 1e-13 * ((xhi - xlo) + (yhi - ylo)) * cm)) continue;
+            if (push_frag(R, (int32_t)v, (int32_t)f, m, px, py)) {
+              R->status = 4;
+              break;
+            }
+          } else if (mode == 0) {
             /* Hessian triplets (kantorovich.hpp:109-122) */
             for (int k = 0; k < m; k++) {
               if (cp[k].type != EDGE_DT) continue;
@@ -1042,8 +1107,27 @@ void orc_copy(const orc_result *r, double *fval, double *g, double *mom, int32_t
 void orc_copy_mom2(const orc_result *r, double *mom2) {
   if (mom2 && r->n > 0) memcpy(mom2, r->mom2, (size_t)(3 * r->n) * sizeof(double));
 }
+int64_t orc_nfrag(const orc_result *r, int64_t *nvert) {
+  if (nvert) *nvert = r ? r->nvx : 0;
+  return r ? r->nf : 0;
+}
+void orc_copy_frags(const orc_result *r, int32_t *cell, int32_t *tri, int64_t *start,
+                    double *xy) {
+  if (!r || r->nf == 0) {
+    if (start) start[0] = 0;
+    return;
+  }
+  memcpy(cell, r->fc, (size_t)r->nf * sizeof(int32_t));
+  memcpy(tri, r->ft, (size_t)r->nf * sizeof(int32_t));
+  memcpy(start, r->fs, (size_t)(r->nf + 1) * sizeof(int64_t));
+  memcpy(xy, r->fxy, (size_t)(2 * r->nvx) * sizeof(double));
+}
 void orc_free(orc_result *r) {
   if (!r) return;
+  free(r->fc);
+  free(r->ft);
+  free(r->fs);
+  free(r->fxy);
   free(r->g);
   free(r->mom);
   free(r->mom2);
@@ -1051,4 +1135,100 @@ void orc_free(orc_result *r) {
   free(r->tj);
   free(r->tv);
   free(r);
+}
+
+/* ====================================================================== */
+/* draw_laguerre_diagram (rasterization.hpp:491-547): fragments from mode  */
+/* 3, each coloured by colors[Dirac] times the mean of rho over its        */
+/* vertices (:528-534) and spread over pixels by exact coverage: the       */
+/* polygon clipped by the pixel square (Sutherland-Hodgman, four axis      */
+/* lines), area by the shoelace formula.  The reference's scan converter   */
+/* (rasterize_convex_polygon, :403-470) computes the same coverage after   */
+/* nudging vertices within 1e-8 of an integer by 1e-6 pixel.              */
+/* ====================================================================== */
+static int clip_axis(const double *ix, const double *iy, int n, int axis, double c, int keep_ge,
+                     double *ox, double *oy) {
+  int m = 0;
+  for (int i = 0; i < n; i++) {
+    int j = (i + 1) % n;
+    double a = axis ? iy[i] : ix[i], b = axis ? iy[j] : ix[j];
+    int ina = keep_ge ? a >= c : a <= c, inb = keep_ge ? b >= c : b <= c;
+    if (ina) {
+      ox[m] = ix[i];
+      oy[m] = iy[i];
+      m++;
+    }
+    if (ina != inb) {
+      double t = (c - a) / (b - a);
+      ox[m] = ix[i] + t * (ix[j] - ix[i]);
+      oy[m] = iy[i] + t * (iy[j] - iy[i]);
+      if (axis) oy[m] = c; else ox[m] = c;
+      m++;
+    }
+  }
+  return m;
+}
+
+int orc_draw_laguerre(int64_t nv, const double *vxy, const double *vval, int64_t nt,
+                      const int32_t *tri, int64_t n, const double *X, const double *w,
+                      const double *colors, int nc, double x0, double y0, double x1,
+                      double y1, int W, int H, double *img) {
+  orc_result *R = orc_run(3, nv, vxy, vval, nt, tri, n, X, w);
+  if (!R) return -1;
+  if (R->status) {
+    int s = R->status;
+    orc_free(R);
+    return s;
+  }
+  const double sx = (double)W / (x1 - x0), sy = (double)H / (y1 - y0);
+  double ax[128], ay[128], bx[128], by[128];
+  for (int64_t f = 0; f < R->nf; f++) {
+    int64_t s0 = R->fs[f];
+    int m = (int)(R->fs[f + 1] - s0);
+    if (m > 100) m = 100; /* longer fragments do not arise in the tests */
+    /* density: the triangle's linear function (functions.hpp:23-80) */
+    otri t;
+    int32_t id[3] = {tri[3 * R->ft[f]], tri[3 * R->ft[f] + 1], tri[3 * R->ft[f] + 2]};
+    double fv[3];
+    for (int k = 0; k < 3; k++) {
+      t.x[k] = vxy[2 * id[k]];
+      t.y[k] = vxy[2 * id[k] + 1];
+      fv[k] = vval[id[k]];
+    }
+    linear_function(&t, fv);
+    double ff = 0.0, px[128], py[128];
+    double lo_x = INFINITY, hi_x = -INFINITY, lo_y = INFINITY, hi_y = -INFINITY;
+    for (int k = 0; k < m; k++) {
+      double x = R->fxy[2 * (s0 + k)], y = R->fxy[2 * (s0 + k) + 1];
+      ff += rho_at(&t, x, y);
+      px[k] = sx * (x - x0);
+      py[k] = sy * (y - y0);
+      lo_x = fmin(lo_x, px[k]); hi_x = fmax(hi_x, px[k]);
+      lo_y = fmin(lo_y, py[k]); hi_y = fmax(hi_y, py[k]);
+    }
+    ff /= (double)m;
+    const double *col = colors + (size_t)R->fc[f] * nc;
+    int i0 = (int)floor(lo_x), i1 = (int)floor(hi_x), j0 = (int)floor(lo_y), j1 = (int)floor(hi_y);
+    if (i0 < 0) i0 = 0;
+    if (j0 < 0) j0 = 0;
+    if (i1 > W - 1) i1 = W - 1;
+    if (j1 > H - 1) j1 = H - 1;
+    for (int j = j0; j <= j1; j++)
+      for (int i = i0; i <= i1; i++) {
+        int q = clip_axis(px, py, m, 0, (double)i, 1, ax, ay);
+        q = clip_axis(ax, ay, q, 0, (double)(i + 1), 0, bx, by);
+        q = clip_axis(bx, by, q, 1, (double)j, 1, ax, ay);
+        q = clip_axis(ax, ay, q, 1, (double)(j + 1), 0, bx, by);
+        double a2 = 0.0;
+        for (int k = 0; k < q; k++) {
+          int k1 = (k + 1) % q;
+          a2 += bx[k] * by[k1] - bx[k1] * by[k];
+        }
+        double cover = 0.5 * a2;
+        if (!(cover > 0.0)) continue;
+        for (int c = 0; c < nc; c++) img[((size_t)j * W + i) * nc + c] += cover * ff * col[c];
+      }
+  }
+  orc_free(R);
+  return 0;
 }

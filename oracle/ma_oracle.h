@@ -11,6 +11,9 @@
  *   include/MA/quadrature.hpp:23-42,69-85,122-143 (centroid, AC6, segment rules)
  *   include/MA/functions.hpp:23-80             (barycentric / Linear_function)
  *   include/MA/lloyd.hpp:30-69                 (first_moment)
+ *   include/MA/lloyd.hpp:71-123                (second_moment)
+ *   include/MA/voronoi_triangulation_intersection.hpp:315-343 (fragment polygons)
+ *   include/MA/rasterization.hpp:403-547       (draw_laguerre_diagram)
  * The CGAL regular triangulation (kantorovich.hpp:64-72) is replaced by a
  * certified FP64 power-cell search that returns a superset of each Dirac's
  * power neighbours; clipping by a superset gives the same fragments.
@@ -26,7 +29,8 @@ extern "C" {
 typedef struct orc_result orc_result;
 
 /* mode 0: kantorovich (fval, g, Hessian triplets); mode 1: first_moment
- * (masses, first moments int rho*x, int rho*y per cell).
+ * (masses, first moments int rho*x, int rho*y per cell); mode 2:
+ * second_moment; mode 3: the polygon of every fragment of positive area.
  * X is n x 2 row-major (x0,y0,x1,y1,...); tri is nt x 3 vertex indices. */
 orc_result *orc_run(int mode, int64_t nv, const double *vxy, const double *vval,
                     int64_t nt, const int32_t *tri, int64_t n, const double *X,
@@ -39,6 +43,23 @@ int64_t orc_ntriplets(const orc_result *r);
 void orc_copy(const orc_result *r, double *fval, double *g, double *mom,
               int32_t *ti, int32_t *tj, double *tv, int64_t *stats);
 void orc_free(orc_result *r);
+void orc_copy_mom2(const orc_result *r, double *mom2);
+/* mode 3 results: fragment count (and total vertex count); fragment f is
+ * Dirac cell[f] cut by triangle tri[f], vertices xy[2*start[f] .. 2*start[f+1])
+ * counter-clockwise.  start has nfrag + 1 entries. */
+int64_t orc_nfrag(const orc_result *r, int64_t *nvert);
+void orc_copy_frags(const orc_result *r, int32_t *cell, int32_t *tri, int64_t *start,
+                    double *xy);
+/* draw_laguerre_diagram (rasterization.hpp:491-547): every fragment, mapped
+ * from the box [x0,x1]x[y0,y1] to a W x H pixel grid, adds
+ * coverage(pixel) * mean of rho over its vertices * colors[cell*nc + c] to
+ * img[(j*W + i)*nc + c], coverage = exact area of fragment cap pixel square
+ * (pixel (i,j) = [i,i+1]x[j,j+1]); pixels outside the grid are dropped.
+ * Returns the oracle status (0 = ok). */
+int orc_draw_laguerre(int64_t nv, const double *vxy, const double *vval, int64_t nt,
+                      const int32_t *tri, int64_t n, const double *X, const double *w,
+                      const double *colors, int nc, double x0, double y0, double x1,
+                      double y1, int W, int H, double *img);
 
 /* Exact sign predicates (exposed for tests).  Each returns the sign of
  * pi_v(x) - pi_w(x) at the implied point x, pi_u(x) = |x-u|^2 - u_w.

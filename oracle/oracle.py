@@ -50,6 +50,13 @@ def _lib():
         lib.orc_free.argtypes = [P]
         lib.orc_copy_mom2.restype = None
         lib.orc_copy_mom2.argtypes = [P, P]
+        lib.orc_nfrag.restype = I64
+        lib.orc_nfrag.argtypes = [P, P]
+        lib.orc_copy_frags.restype = None
+        lib.orc_copy_frags.argtypes = [P, P, P, P, P]
+        lib.orc_draw_laguerre.restype = ctypes.c_int
+        lib.orc_draw_laguerre.argtypes = ([I64, P, P, I64, P, I64, P, P, P, ctypes.c_int]
+                                          + [ctypes.c_double] * 4 + [ctypes.c_int] * 2 + [P])
         for nm in ("orc_side1", "orc_side1_exact"):
             f = getattr(lib, nm)
             f.restype = ctypes.c_int
@@ -158,6 +165,68 @@ def second_moment(mesh, X, w):
     int rho x y), ``include/MA/lloyd.hpp:71-123``."""
     _, m, (mom, mom2), _, _ = _run(2, mesh, X, w)
     return m, mom, mom2
+
+
+def fragments(mesh, X, w):
+    """Polygon of every non-empty fragment (Laguerre cell of a Dirac cut by a
+    triangle): the traversal ``voronoi_triangulation_intersection``
+    (``include/MA/voronoi_triangulation_intersection.hpp:315-343``) with a
+    callback that records ``(polygon, face, vertex)``.  Returns
+    ``(cell, tri, start, xy)``: fragment f is Dirac ``cell[f]`` cut by
+    triangle ``tri[f]``, vertices ``xy[start[f]:start[f+1]]`` counter-clockwise,
+    in the order the oracle visits them (Dirac by Dirac)."""
+    lib = _lib()
+    vxy = _c(mesh.vxy, np.float64)
+    vval = _c(mesh.vval, np.float64)
+    tri = _c(mesh.tri, np.int32)
+    X = _c(X, np.float64).reshape(-1, 2)
+    w = _c(w, np.float64).reshape(-1)
+    n = X.shape[0]
+    r = lib.orc_run(3, vxy.shape[0], _ptr(vxy), _ptr(vval), tri.shape[0], _ptr(tri), n,
+                    _ptr(X), _ptr(w))
+    if not r:
+        raise MemoryError("oracle allocation failed")
+    try:
+        st = lib.orc_status(r)
+        if st != 0:
+            raise RuntimeError(f"oracle failed with status {st}")
+        nvx = np.zeros(1, np.int64)
+        nf = lib.orc_nfrag(r, _ptr(nvx))
+        cell = np.zeros(max(nf, 1), np.int32)
+        ft = np.zeros(max(nf, 1), np.int32)
+        start = np.zeros(nf + 1, np.int64)
+        xy = np.zeros((max(int(nvx[0]), 1), 2))
+        lib.orc_copy_frags(r, _ptr(cell), _ptr(ft), _ptr(start), _ptr(xy))
+    finally:
+        lib.orc_free(r)
+    return cell[:nf], ft[:nf], start, xy[:int(nvx[0])]
+
+
+def draw_laguerre_diagram(mesh, X, w, colors, box, width, height):
+    """``draw_laguerre_diagram`` (``include/MA/rasterization.hpp:491-547``)
+    with a summing ``put_pixel``: image ``(height, width, channels)`` where
+    pixel (i, j) (row j counted from ``box``'s lower edge) receives, from
+    every fragment, its exact coverage of the pixel times the mean of the
+    density over the fragment's vertices times ``colors[cell]``."""
+    lib = _lib()
+    vxy = _c(mesh.vxy, np.float64)
+    vval = _c(mesh.vval, np.float64)
+    tri = _c(mesh.tri, np.int32)
+    X = _c(X, np.float64).reshape(-1, 2)
+    w = _c(w, np.float64).reshape(-1)
+    col = np.asarray(colors, np.float64)
+    if col.ndim == 1:
+        col = col[:, None]
+    col = np.ascontiguousarray(col)
+    nc = col.shape[1]
+    img = np.zeros((height, width, nc))
+    st = lib.orc_draw_laguerre(vxy.shape[0], _ptr(vxy), _ptr(vval), tri.shape[0], _ptr(tri),
+                               X.shape[0], _ptr(X), _ptr(w), _ptr(col), nc, float(box[0]),
+                               float(box[1]), float(box[2]), float(box[3]), int(width),
+                               int(height), _ptr(img))
+    if st != 0:
+        raise RuntimeError(f"oracle failed with status {st}")
+    return img
 
 
 def lloyd(mesh, X, w):

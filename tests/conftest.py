@@ -35,7 +35,27 @@ def load_golden(name):
         pattern.add((i, j))
     return dict(name=name, mesh=mesh, X=X, w=w, fval=e["fval"], mass=np.array(e["mass"]),
                 intg=np.array(e["intg"]), mom=np.array(e["mom"]), H=H, pattern=pattern,
-                inertia=np.array(e["inertia"]))
+                inertia=np.array(e["inertia"]), frags=np.array(e["fragments"]).reshape(-1, 5))
+
+
+def frag_summary(cell, tri, start, xy):
+    """{(Dirac, triangle): (area, centroid x, centroid y)} of a fragment list
+    (vertices xy[start[f]:start[f+1]], counter-clockwise), by the shoelace
+    formula about the first vertex."""
+    out = {}
+    for f in range(len(cell)):
+        p = xy[start[f]:start[f + 1]]
+        q = p - p[0]
+        x0, y0 = q[1:-1, 0], q[1:-1, 1]
+        x1, y1 = q[2:, 0], q[2:, 1]
+        a = 0.5 * (x0 * y1 - x1 * y0)
+        A = a.sum()
+        cx = (a * (x0 + x1)).sum() / (3 * A) + p[0, 0]
+        cy = (a * (y0 + y1)).sum() / (3 * A) + p[0, 1]
+        key = (int(cell[f]), int(tri[f]))
+        assert key not in out, f"fragment {key} listed twice"
+        out[key] = (A, cx, cy)
+    return out
 
 
 GOLDEN_CASES = sorted(f[:-5] for f in os.listdir(GOLDEN) if f.endswith(".json"))

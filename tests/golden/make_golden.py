@@ -99,7 +99,8 @@ def solve(vxy, vval, tri, X, w):
     syy = [Fr(0)] * n
     sxy = [Fr(0)] * n
     hess = {}
-    for t in tri:
+    frags = []
+    for ti, t in enumerate(tri):
         p = [V[k] for k in t]
         if area(*p) < 0:
             p = [p[0], p[2], p[1]]
@@ -122,6 +123,19 @@ def solve(vxy, vval, tri, X, w):
                     break
             if len(poly) < 3:
                 continue
+            # the fragment's exact area and centroid (fragments of zero area
+            # are not listed)
+            fa = Fr(0)
+            fcx = Fr(0)
+            fcy = Fr(0)
+            for k in range(1, len(poly) - 1):
+                a, b, c = poly[0], poly[k], poly[k + 1]
+                ak = area(a, b, c)
+                fa += ak
+                fcx += ak * (a[0] + b[0] + c[0]) / 3
+                fcy += ak * (a[1] + b[1] + c[1]) / 3
+            if fa > 0:
+                frags.append([i, ti, float(fa), float(fcx / fa), float(fcy / fa)])
             yi = Y[i]
             for k in range(1, len(poly) - 1):
                 a, b, c = poly[0], poly[k], poly[k + 1]
@@ -155,6 +169,7 @@ def solve(vxy, vval, tri, X, w):
         inertia=[[float(sxx[i]), float(syy[i]), float(sxy[i])] for i in range(n)],
         total=float(sum(mass)),
         hessian=[[i, j, float(v)] for (i, j), v in sorted(hess.items())],
+        fragments=sorted(frags),
     )
 
 
