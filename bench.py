@@ -77,6 +77,50 @@ def cpu_baseline_sample(p, frac=0.5):
     }
 
 
+def bench_outputs(e, p, N, dev, torch, reps=3):
+    """The output entry points on the same workload at w = 0: fragment
+    polygons to host arrays (MA::voronoi_triangulation_intersection), the
+    Laguerre diagram rasterised into a 1024 x 1024 x 3 image held on the
+    device (MA::draw_laguerre_diagram), and a 1024 x 1024 image turned into
+    the P1 density mesh (MA::image_to_pl_function)."""
+    from mongeampere_amd.api import Engine
+
+    res = {}
+    w_host = np.zeros(N)
+    log("fragment polygons")
+    cell, _, _, xy = e.fragments(w_host)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        cell, _, _, xy = e.fragments(w_host)
+    dt = (time.perf_counter() - t0) / reps
+    res["fragments_ms_host_arrays"] = dt * 1e3
+    res["fragments"] = int(cell.shape[0])
+    res["fragment_vertices"] = int(xy.shape[0])
+    log("rasterised diagram")
+    W = H = 1024
+    w_dev = torch.zeros(N, dtype=torch.float64, device=dev)
+    col = torch.rand((N, 3), dtype=torch.float64, device=dev, generator=None)
+    img = torch.empty((H, W, 3), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    box = (0.0, 0.0, 1.0, 1.0)
+    e.draw_laguerre_device(w_dev.data_ptr(), col.data_ptr(), 3, box, W, H, img.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e.draw_laguerre_device(w_dev.data_ptr(), col.data_ptr(), 3, box, W, H, img.data_ptr())
+    torch.cuda.synchronize()
+    res["draw_laguerre_ms_1024x1024x3_device"] = (time.perf_counter() - t0) / reps * 1e3
+    log("image density mesh")
+    pix = np.random.default_rng(0).integers(0, 256, (1024, 1024)).astype(np.float64)
+    with Engine(0) as e2:
+        e2.set_mesh_image(pix)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e2.set_mesh_image(pix)
+        res["image_mesh_ms_1024x1024_host_pixels"] = (time.perf_counter() - t0) / reps * 1e3
+    return res
+
+
 def time_evals(e, w_ptr, precision, steps):
     """Times `steps` evaluations (host wall clock between synchronisations)
     and collects the per-phase device times of each."""
@@ -108,6 +152,8 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-outputs", action="store_true",
+                    help="skip the fragment / rasterisation / image-mesh timings")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the evaluation rate of the other precision")
     ap.add_argument("--N", type=int, default=1_000_000)
@@ -208,6 +254,8 @@ def main():
         for _ in range(nh):
             e.second_moment(w_host)
         out["second_moment_per_s_w0_host_buffers"] = nh / (time.perf_counter() - t0)
+    if not args.no_outputs:
+        out["outputs"] = bench_outputs(e, p, N, dev, torch)
     if not args.no_solve:
         w_star = None
         for q in [q for q in args.solve_precisions.split(",") if q]:

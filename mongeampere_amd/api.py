@@ -284,18 +284,22 @@ class Engine:
         w = np.ascontiguousarray(w, np.float64)
         nf = ctypes.c_int64()
         nvx = ctypes.c_int64()
-        st = self._L.ma_fragments(self._h, _ptr(w), 0, 0, ctypes.byref(nf), ctypes.byref(nvx),
-                                  None, None, None, None)
-        if st not in (0, _lib.MA_CAPACITY):
-            self._check(st)
-        cell = np.empty(max(nf.value, 1), np.int32)
-        tri = np.empty(max(nf.value, 1), np.int32)
-        start = np.empty(nf.value + 1, np.int64)
-        xy = np.empty((max(nvx.value, 1), 2))
-        self._check(self._L.ma_fragments(self._h, _ptr(w), nf.value, nvx.value,
-                                         ctypes.byref(nf), ctypes.byref(nvx), _ptr(cell),
-                                         _ptr(tri), _ptr(start), _ptr(xy)))
-        return cell[:nf.value], tri[:nf.value], start, xy[:nvx.value]
+        # arrays sized from the previous call (one pass when they suffice)
+        cf, cv = getattr(self, "_frag_cap", (0, 0))
+        for _ in range(2):
+            cell = np.empty(max(cf, 1), np.int32)
+            tri = np.empty(max(cf, 1), np.int32)
+            start = np.empty(cf + 1, np.int64)
+            xy = np.empty((max(cv, 1), 2))
+            st = self._L.ma_fragments(self._h, _ptr(w), cf, cv, ctypes.byref(nf),
+                                      ctypes.byref(nvx), _ptr(cell), _ptr(tri), _ptr(start),
+                                      _ptr(xy))
+            if st != _lib.MA_CAPACITY:
+                break
+            cf, cv = nf.value, nvx.value
+        self._check(st)
+        self._frag_cap = (max(cf, nf.value), max(cv, nvx.value))
+        return cell[:nf.value], tri[:nf.value], start[:nf.value + 1], xy[:nvx.value]
 
     def draw_laguerre_diagram(self, w, colors, box, width, height):
         """``MA::draw_laguerre_diagram`` (include/MA/rasterization.hpp:491-547)

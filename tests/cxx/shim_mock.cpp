@@ -5,7 +5,9 @@
 // Input (text file): nv, then nv lines "x y value"; nt, then nt lines
 // "i j k"; n, then n lines "x y nu".  Output: one JSON object with the
 // results of MA::kantorovich (w = 0), MA::ot_solve, MA::solve_laplacian_matrix
-// MA::lloyd and MA::second_moment.
+// MA::lloyd, MA::second_moment, MA::voronoi_triangulation_intersection
+// (fragment count and area per Dirac) and MA::draw_laguerre_diagram (a 16 x 12
+// image of the unit square, colour of Dirac i = i + 1).
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -15,6 +17,8 @@
 #include "MA/kantorovich.hpp"
 #include "MA/lloyd.hpp"
 #include "MA/optimal_transport.hpp"
+#include "MA/rasterization.hpp"
+#include "MA/voronoi_triangulation_intersection.hpp"
 
 namespace mock {
 
@@ -167,7 +171,33 @@ int main(int argc, char **argv) {
   mock::Matrix mom, inertia;
   MA::second_moment(T, F, X, x, m2, mom, inertia);
 
+  std::vector<double> farea(n, 0.0);
+  int64_t nfrag = 0;
+  MA::voronoi_triangulation_intersection(
+      T, F, X, x, [&](const std::vector<mock::Point> &poly, mock::Face_handle f, size_t i) {
+        (void)f;
+        double a = 0.0;
+        for (size_t k = 0; k < poly.size(); k++) {
+          const mock::Point &p = poly[k], &q = poly[(k + 1) % poly.size()];
+          a += p.x() * q.y() - q.x() * p.y();
+        }
+        farea[i] += 0.5 * a;
+        nfrag++;
+      });
+  std::vector<double> colors(n);
+  for (int64_t i = 0; i < n; i++) colors[i] = (double)(i + 1);
+  const int IW = 16, IH = 12;
+  std::vector<double> image(IW * IH, 0.0);
+  MA::draw_laguerre_diagram(T, F, X, x, colors, 0.0, 0.0, 1.0, 1.0, IW, IH,
+                            [&](int px, int py, double v) { image[py * IW + px] += v; });
+
   std::printf("{\"fval\": %.17g, \"niter\": %zu, \"neval\": %zu,\n", fval, st.niter, st.neval);
+  std::printf("\"nfrag\": %lld,\n", (long long)nfrag);
+  std::printf("\"frag_area\": [");
+  for (int64_t i = 0; i < n; i++) std::printf("%s%.17g", i ? ", " : "", farea[i]);
+  std::printf("],\n\"image\": [");
+  for (int k = 0; k < IW * IH; k++) std::printf("%s%.17g", k ? ", " : "", image[k]);
+  std::printf("],\n");
   auto vec = [&](const char *name, const mock::Vector &v, bool last = false) {
     std::printf("\"%s\": [", name);
     for (int64_t i = 0; i < v.size(); i++) std::printf("%s%.17g", i ? ", " : "", v(i));

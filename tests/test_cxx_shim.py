@@ -1,5 +1,6 @@
 """The header-only C++ shim (include/MA/*.hpp) keeps the reference's call
-shapes.  CPU: it compiles and links against stand-ins of the CGAL / Eigen
+shapes (kantorovich, ot_solve, solve_laplacian_matrix, lloyd, second_moment,
+voronoi_triangulation_intersection, draw_laguerre_diagram).  CPU: it compiles and links against stand-ins of the CGAL / Eigen
 types.  GPU: the prebuilt driver program gives the same results as the
 Python binding of the C ABI on the same inputs."""
 import json
@@ -68,3 +69,14 @@ def test_shim_matches_c_abi(engine, tmp_path):
     assert np.max(np.abs(np.array(out["centroids"]) - c)) <= 1e-12
     _, _, inertia = engine.second_moment(w)
     assert np.max(np.abs(np.array(out["inertia"]) - inertia)) <= 1e-14
+    # traversal callback and rasterisation at the driver's solved weights
+    wm = np.array(out["w"])
+    cell, tri, start, xy = engine.fragments(wm)
+    assert out["nfrag"] == cell.shape[0]
+    area = np.zeros(N)
+    for f in range(cell.shape[0]):
+        q = xy[start[f]:start[f + 1]]
+        area[cell[f]] += 0.5 * np.sum(q[:, 0] * np.roll(q[:, 1], -1) - np.roll(q[:, 0], -1) * q[:, 1])
+    assert np.max(np.abs(np.array(out["frag_area"]) - area)) <= 1e-14
+    img = engine.draw_laguerre_diagram(wm, np.arange(1.0, N + 1.0), (0, 0, 1, 1), 16, 12)
+    assert np.max(np.abs(np.array(out["image"]) - img[..., 0].ravel())) <= 1e-12 * img.max()
