@@ -81,8 +81,10 @@ def bench_outputs(e, p, N, dev, torch, reps=3):
     """The output entry points on the same workload at w = 0: fragment
     polygons to host arrays (MA::voronoi_triangulation_intersection), the
     Laguerre diagram rasterised into a 1024 x 1024 x 3 image held on the
-    device (MA::draw_laguerre_diagram), and a 1024 x 1024 image turned into
-    the P1 density mesh (MA::image_to_pl_function)."""
+    device (MA::draw_laguerre_diagram), the cells of 100000 uniform Diracs
+    clipped to the square [-1,1]^2 (MA::voronoi_polygon_intersection, the
+    workload of the reference's tests/test_power.cpp), and a 1024 x 1024
+    image turned into the P1 density mesh (MA::image_to_pl_function)."""
     from mongeampere_amd.api import Engine
 
     res = {}
@@ -110,6 +112,16 @@ def bench_outputs(e, p, N, dev, torch, reps=3):
         e.draw_laguerre_device(w_dev.data_ptr(), col.data_ptr(), 3, box, W, H, img.data_ptr())
     torch.cuda.synchronize()
     res["draw_laguerre_ms_1024x1024x3_device"] = (time.perf_counter() - t0) / reps * 1e3
+    log("cells clipped to a square")
+    sq = np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]])
+    Xs = np.random.default_rng(1).uniform(-1, 1, (100000, 2))
+    with Engine(0) as e3:
+        e3.set_mesh_polygon(sq).set_diracs(Xs)
+        e3.cell_polygons(np.zeros(100000))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e3.cell_polygons(np.zeros(100000))
+        res["cell_polygons_ms_square_N100000_host_arrays"] = (time.perf_counter() - t0) / reps * 1e3
     log("image density mesh")
     pix = np.random.default_rng(0).integers(0, 256, (1024, 1024)).astype(np.float64)
     with Engine(0) as e2:

@@ -199,6 +199,25 @@ ma_status ma_draw_laguerre(ma_engine *e, const double *w, const double *colors, 
                            double x0, double y0, double x1, double y1, int width, int height,
                            double *img, int device_ptrs);
 
+/* Uniform density `density` on a strictly convex polygon (m vertices,
+ * interleaved x, y, either orientation), triangulated as a fan from its first
+ * vertex: the domain of MA::voronoi_polygon_intersection
+ * (include/MA/voronoi_polygon_intersection.hpp:153-188). */
+ma_status ma_set_mesh_polygon(ma_engine *e, int m, const double *pxy, double density);
+
+/* Every Dirac's Laguerre cell clipped to the mesh domain, replacing
+ * MA::voronoi_polygon_intersection(P, dt, v)
+ * (include/MA/voronoi_polygon_intersection.hpp:153-188, itself
+ * polygon_halfplane_intersection of include/MA/polygon_intersection.hpp:46-73
+ * over the neighbours of v) for all v at once: cell i has vertices
+ * xy[2*start[i] .. 2*start[i+1]) (counter-clockwise, no collinear or repeated
+ * vertices; none for an empty cell); start has n + 1 entries.  Exact for a
+ * convex domain (ma_set_mesh_polygon, grid meshes): the cell polygon is the
+ * convex hull of the cell's fragments.  *nvert always receives the total; if
+ * cap_vert < *nvert nothing is copied and MA_CAPACITY is returned. */
+ma_status ma_cell_polygons(ma_engine *e, const double *w, int64_t cap_vert, int64_t *nvert,
+                           int64_t *start, double *xy);
+
 /* Mesh and density from a grey-level image, replacing MA::image_to_pl_function
  * (include/MA/functions.hpp:82-120): pixels is width x height, row-major with
  * the first row at the top (pixel (x, y) at pixels[y * width + x], the layout

@@ -63,6 +63,8 @@ SIGNATURES = [
     ("ma_fragments", I32, [P, P, I64, I64, P, P, P, P, P, P]),
     ("ma_draw_laguerre", I32, [P, P, P, I32, F64, F64, F64, F64, I32, I32, P, I32]),
     ("ma_set_mesh_image", I32, [P, I32, I32, P, P]),
+    ("ma_set_mesh_polygon", I32, [P, I32, P, F64]),
+    ("ma_cell_polygons", I32, [P, P, I64, P, P, P]),
     ("ma_adjoint", I32, [P, P, P, P]),
     ("ma_get_eval_info", I32, [P, P]),
     ("ma_cell_paths", I32, [P, I64, P, P]),

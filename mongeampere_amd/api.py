@@ -17,6 +17,9 @@ Reference entry points and their counterparts here:
                                                 -> ``draw_laguerre_diagram``
 * ``MA::image_to_pl_function`` (``include/MA/functions.hpp:82-120``)
                                                 -> ``Engine.set_mesh_image``
+* ``MA::voronoi_polygon_intersection(P, dt, v)``
+  (``include/MA/voronoi_polygon_intersection.hpp:153-188``)
+                                                -> ``voronoi_polygon_intersection``
 
 Everything runs on the GPU through ``libmagpu.so``; this module only moves
 arrays across the C ABI.
@@ -275,6 +278,35 @@ class Engine:
         self._check(self._L.ma_adjoint(self._h, _ptr(w), _ptr(dY), _ptr(dval)))
         return dY, dval
 
+    def set_mesh_polygon(self, P, density=1.0):
+        """Uniform density on a strictly convex polygon (k x 2 vertices), the
+        domain of ``MA::voronoi_polygon_intersection``."""
+        P = np.ascontiguousarray(P, np.float64).reshape(-1, 2)
+        self._check(self._L.ma_set_mesh_polygon(self._h, P.shape[0], _ptr(P), float(density)))
+        self.mesh = None
+        self.nv = P.shape[0]
+        return self
+
+    def cell_polygons(self, w):
+        """Every Dirac's Laguerre cell clipped to the (convex) mesh domain
+        (include/MA/voronoi_polygon_intersection.hpp:153-188 for all Diracs):
+        ``(start, xy)``, cell i with vertices ``xy[start[i]:start[i+1]]``
+        counter-clockwise (none for an empty cell)."""
+        w = np.ascontiguousarray(w, np.float64)
+        nv = ctypes.c_int64()
+        start = np.empty(self.n + 1, np.int64)
+        cap = getattr(self, "_poly_cap", 8 * self.n)
+        for _ in range(2):
+            xy = np.empty((max(cap, 1), 2))
+            st = self._L.ma_cell_polygons(self._h, _ptr(w), cap, ctypes.byref(nv), _ptr(start),
+                                          _ptr(xy))
+            if st != _lib.MA_CAPACITY:
+                break
+            cap = nv.value
+        self._check(st)
+        self._poly_cap = max(cap, nv.value)
+        return start, xy[:nv.value]
+
     # ---- fragments / rasterisation ---------------------------------------
     def fragments(self, w):
         """Every fragment (Laguerre cell cut by a triangle) of positive area,
@@ -393,3 +425,13 @@ def image_to_pl_function(image, engine=None):
     engine (a new one by default); returns ``(engine, total_mass)``."""
     e = engine if engine is not None else Engine()
     return e, e.set_mesh_image(image)
+
+
+def voronoi_polygon_intersection(P, X, weights, engine=None):
+    """``MA::voronoi_polygon_intersection`` for every Dirac: the Laguerre cell
+    of each Dirac (X, weights) clipped to the strictly convex polygon P;
+    returns a list of (k, 2) arrays (empty for an empty cell)."""
+    e = engine if engine is not None else Engine()
+    e.set_mesh_polygon(P).set_diracs(X)
+    start, xy = e.cell_polygons(weights)
+    return [xy[start[i]:start[i + 1]] for i in range(len(start) - 1)]

@@ -119,6 +119,9 @@ struct OutBuffers {
   DBuf<int32_t> scell, stri;
   DBuf<double> colors, img;   // rasterisation
   DBuf<double> pix;           // image pixels (mesh from an image)
+  DBuf<int64_t> vbeg, vend;   // per Dirac: its fragments' vertices in sorted order
+  DBuf<int64_t> hcnt, hoff;   // per Dirac: cell polygon vertex count and offset
+  DBuf<double> hxy;           // cell polygons
 };
 
 struct EvalBuffers {

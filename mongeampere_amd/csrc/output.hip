@@ -315,6 +315,89 @@ __global__ void __launch_bounds__(256) k_raster_big(RasterArgs r) {
 }
 
 // ---------------------------------------------------------------------------
+// Cells clipped to a convex domain (voronoi_polygon_intersection)
+// ---------------------------------------------------------------------------
+
+// Per Dirac, the range of its fragments' vertices in the sorted arrays.
+__global__ void k_cell_ranges(int64_t nf, const int32_t *cell, const int64_t *start,
+                              int64_t *vbeg, int64_t *vend) {
+  const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int32_t c = cell[f];
+  if (f == 0 || cell[f - 1] != c) vbeg[c] = start[f];
+  if (f == nf - 1 || cell[f + 1] != c) vend[c] = start[f + 1];
+}
+
+// Counter-clockwise convex hull of the points xy[b..e) by gift wrapping
+// (Jarvis march), written to out (when not null); returns its vertex count.
+// Over a convex domain the fragments of a cell tile cell cap domain, a
+// convex polygon whose vertices are among the fragments' vertices; the other
+// fragment vertices lie on its edges or inside.  Points within 1e-12 of the
+// extent count as one, turns below 1e-12 of the extent squared as straight.
+__device__ int hull_points(const double *xy, int64_t b, int64_t e, double *out) {
+  if (e - b < 3) return 0;
+  double xlo = INFINITY, xhi = -INFINITY, ylo = INFINITY, yhi = -INFINITY;
+  int64_t p0 = b;
+  for (int64_t q = b; q < e; q++) {
+    const double x = xy[2 * q], y = xy[2 * q + 1];
+    xlo = fmin(xlo, x);
+    xhi = fmax(xhi, x);
+    ylo = fmin(ylo, y);
+    yhi = fmax(yhi, y);
+    const double x0 = xy[2 * p0], y0 = xy[2 * p0 + 1];
+    if (y < y0 || (y == y0 && x < x0)) p0 = q;
+  }
+  const double ext = fmax(xhi - xlo, yhi - ylo);
+  const double tol = 1e-12 * ext, eps = 1e-12 * ext * ext;
+  int64_t cur = p0;
+  int cnt = 0;
+  const int64_t limit = e - b;
+  do {
+    const double cx = xy[2 * cur], cy = xy[2 * cur + 1];
+    if (out) {
+      out[2 * cnt] = cx;
+      out[2 * cnt + 1] = cy;
+    }
+    cnt++;
+    int64_t cand = -1;
+    double ax = 0.0, ay = 0.0;
+    for (int64_t q = b; q < e; q++) {
+      const double rx = xy[2 * q] - cx, ry = xy[2 * q + 1] - cy;
+      if (fabs(rx) <= tol && fabs(ry) <= tol) continue;  // the current point
+      if (cand < 0) {
+        cand = q;
+        ax = rx;
+        ay = ry;
+        continue;
+      }
+      const double cr = ax * ry - ay * rx;
+      if (cr < -eps || (cr <= eps && rx * rx + ry * ry > ax * ax + ay * ay)) {
+        cand = q;
+        ax = rx;
+        ay = ry;
+      }
+    }
+    if (cand < 0) return 0;  // all points coincide
+    cur = cand;
+  } while ((fabs(xy[2 * cur] - xy[2 * p0]) > tol || fabs(xy[2 * cur + 1] - xy[2 * p0 + 1]) > tol) &&
+           cnt < limit);
+  return cnt >= 3 ? cnt : 0;
+}
+
+__global__ void k_hull_count(int64_t n, const double *xy, const int64_t *vbeg,
+                             const int64_t *vend, int64_t *cnt) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) cnt[i] = hull_points(xy, vbeg[i], vend[i], nullptr);
+  if (i == n) cnt[i] = 0;
+}
+
+__global__ void k_hull_write(int64_t n, const double *xy, const int64_t *vbeg,
+                             const int64_t *vend, const int64_t *off, double *out) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n && off[i + 1] > off[i]) hull_points(xy, vbeg[i], vend[i], out + 2 * off[i]);
+}
+
+// ---------------------------------------------------------------------------
 // P1 density of a grey-level image
 // ---------------------------------------------------------------------------
 
@@ -441,6 +524,98 @@ ma_status ma_draw_laguerre(ma_engine *e, const double *w, const double *colors, 
   MA_HIP_TRY(hipGetLastError());
   MA_HIP_TRY(hipStreamSynchronize(e->stream));
   return MA_OK;
+}
+
+ma_status ma_cell_polygons(ma_engine *e, const double *w, int64_t cap_vert, int64_t *nvert,
+                           int64_t *start, double *xy) {
+  ma_status s = need(e);
+  if (s != MA_OK) return s;
+  if (!w || !nvert || cap_vert < 0) return MA_BAD_INPUT;
+  hipSetDevice(e->device);
+  const int64_t n = e->n;
+  MA_HIP_TRY(e->eb.w_s.ensure((size_t)n));
+  MA_HIP_TRY(to_morton(e, w, e->eb.w_s.p, false));
+  int64_t nf = 0, nvx = 0;
+  s = emit_fragments(e, &nf, &nvx);
+  if (s != MA_OK) return s;
+  if (nf > INT32_MAX) {
+    e->err = "more than 2^31 fragments";
+    return MA_CAPACITY;
+  }
+  s = sort_fragments(e, nf, nvx);
+  if (s != MA_OK) return s;
+  OutBuffers &O = e->ob;
+  MA_HIP_TRY(O.vbeg.ensure((size_t)n));
+  MA_HIP_TRY(O.vend.ensure((size_t)n));
+  MA_HIP_TRY(O.hcnt.ensure((size_t)n + 1));
+  MA_HIP_TRY(O.hoff.ensure((size_t)n + 1));
+  MA_HIP_TRY(hipMemsetAsync(O.vbeg.p, 0, sizeof(int64_t) * n, e->stream));
+  MA_HIP_TRY(hipMemsetAsync(O.vend.p, 0, sizeof(int64_t) * n, e->stream));
+  if (nf > 0)
+    hipLaunchKernelGGL(k_cell_ranges, dim3(nblocks(nf)), dim3(kBlock), 0, e->stream, nf,
+                       O.scell.p, O.sstart.p, O.vbeg.p, O.vend.p);
+  hipLaunchKernelGGL(k_hull_count, dim3(nblocks(n + 1)), dim3(kBlock), 0, e->stream, n, O.sxy.p,
+                     O.vbeg.p, O.vend.p, O.hcnt.p);
+  size_t scan_bytes = 0;
+  MA_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, O.hcnt.p, O.hoff.p,
+                                              (int)(n + 1), e->stream));
+  MA_HIP_TRY(O.sort_tmp.ensure(scan_bytes));
+  MA_HIP_TRY(hipcub::DeviceScan::ExclusiveSum(O.sort_tmp.p, scan_bytes, O.hcnt.p, O.hoff.p,
+                                              (int)(n + 1), e->stream));
+  int64_t total = 0;
+  MA_HIP_TRY(hipMemcpyAsync(&total, O.hoff.p + n, sizeof(int64_t), hipMemcpyDeviceToHost,
+                            e->stream));
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  *nvert = total;
+  if (total > cap_vert) {
+    e->err = "cell polygon array too small (size returned in nvert)";
+    return MA_CAPACITY;
+  }
+  if (!start || (total > 0 && !xy)) return MA_BAD_INPUT;
+  MA_HIP_TRY(O.hxy.ensure((size_t)std::max<int64_t>(2 * total, 1)));
+  hipLaunchKernelGGL(k_hull_write, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, n, O.sxy.p,
+                     O.vbeg.p, O.vend.p, O.hoff.p, O.hxy.p);
+  MA_HIP_TRY(hipGetLastError());
+  MA_HIP_TRY(hipMemcpyAsync(start, O.hoff.p, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost,
+                            e->stream));
+  if (total > 0)
+    MA_HIP_TRY(hipMemcpyAsync(xy, O.hxy.p, sizeof(double) * 2 * total, hipMemcpyDeviceToHost,
+                              e->stream));
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  return MA_OK;
+}
+
+ma_status ma_set_mesh_polygon(ma_engine *e, int m, const double *pxy, double density) {
+  if (!e || m < 3 || !pxy || !(density > 0) || !std::isfinite(density)) return MA_BAD_INPUT;
+  std::vector<double> v(pxy, pxy + 2 * m);
+  double a2 = 0.0;
+  for (int i = 0; i < m; i++) {
+    const int j = (i + 1) % m;
+    a2 += v[2 * i] * v[2 * j + 1] - v[2 * j] * v[2 * i + 1];
+  }
+  if (a2 < 0) {  // clockwise: reverse
+    for (int i = 0; i < m / 2; i++) {
+      std::swap(v[2 * i], v[2 * (m - 1 - i)]);
+      std::swap(v[2 * i + 1], v[2 * (m - 1 - i) + 1]);
+    }
+  }
+  for (int i = 0; i < m; i++) {
+    const int j = (i + 1) % m, k = (i + 2) % m;
+    const double cr = (v[2 * j] - v[2 * i]) * (v[2 * k + 1] - v[2 * j + 1]) -
+                      (v[2 * j + 1] - v[2 * i + 1]) * (v[2 * k] - v[2 * j]);
+    if (!(cr > 0)) {
+      e->err = "polygon is not strictly convex";
+      return MA_BAD_INPUT;
+    }
+  }
+  std::vector<double> val((size_t)m, density);
+  std::vector<int32_t> tri;
+  for (int i = 1; i + 1 < m; i++) {
+    tri.push_back(0);
+    tri.push_back(i);
+    tri.push_back(i + 1);
+  }
+  return ma_set_mesh(e, m, v.data(), val.data(), m - 2, tri.data());
 }
 
 ma_status ma_set_mesh_image(ma_engine *e, int width, int height, const double *pixels,

@@ -4,4 +4,5 @@ cpu_baseline leg of bench.py."""
 from .oracle import (OracleStats, SolveStats, ac6_poly, centroid_lin, draw_laguerre_diagram,
                      first_moment, fragments,
                      kantorovich, kantorovich_triplets, lloyd, ot_solve, second_moment, side1, side1_exact,
-                     side2, side2_exact, side3, side3_exact, solve_laplacian_matrix)
+                     side2, side2_exact, side3, side3_exact, solve_laplacian_matrix,
+                     voronoi_polygon_intersection)

@@ -229,6 +229,61 @@ def draw_laguerre_diagram(mesh, X, w, colors, box, width, height):
     return img
 
 
+def voronoi_polygon_intersection(P, X, w):
+    """Laguerre cell of every Dirac clipped to the polygon P, in exact
+    rational arithmetic (small inputs): ``voronoi_polygon_intersection``
+    (``include/MA/voronoi_polygon_intersection.hpp:153-188``) clips P by the
+    power half-plane of every RT neighbour of v with Sutherland-Hodgman
+    (``polygon_halfplane_intersection``, ``include/MA/polygon_intersection.hpp:46-73``);
+    clipping by all other Diracs (a superset of the neighbours) gives the same
+    polygon.  Half-plane of j: 2 x.(y_j - y_i) <= |y_j|^2 - |y_i|^2 - w_j + w_i.
+    Repeated and collinear vertices are removed from the result.  Returns a
+    list of (k, 2) float arrays, counter-clockwise."""
+    from fractions import Fraction as Fr
+
+    Pq = [(Fr(float(x)), Fr(float(y))) for x, y in np.asarray(P, np.float64).reshape(-1, 2)]
+    a2 = sum(Pq[i][0] * Pq[(i + 1) % len(Pq)][1] - Pq[(i + 1) % len(Pq)][0] * Pq[i][1]
+             for i in range(len(Pq)))
+    if a2 < 0:
+        Pq = Pq[::-1]
+    Y = [(Fr(float(x)), Fr(float(y))) for x, y in np.asarray(X, np.float64).reshape(-1, 2)]
+    W = [Fr(float(v)) for v in np.asarray(w, np.float64).reshape(-1)]
+    out = []
+    for i in range(len(Y)):
+        poly = list(Pq)
+        for j in range(len(Y)):
+            if j == i or not poly:
+                continue
+            ax, ay = 2 * (Y[j][0] - Y[i][0]), 2 * (Y[j][1] - Y[i][1])
+            c = (Y[j][0] ** 2 + Y[j][1] ** 2) - (Y[i][0] ** 2 + Y[i][1] ** 2) - W[j] + W[i]
+            f = [ax * p[0] + ay * p[1] - c for p in poly]  # <= 0 inside
+            nxt = []
+            for k in range(len(poly)):
+                k1 = (k + 1) % len(poly)
+                if f[k] <= 0:
+                    nxt.append(poly[k])
+                if (f[k] < 0 < f[k1]) or (f[k1] < 0 < f[k]):
+                    t = f[k] / (f[k] - f[k1])
+                    nxt.append((poly[k][0] + t * (poly[k1][0] - poly[k][0]),
+                                poly[k][1] + t * (poly[k1][1] - poly[k][1])))
+            poly = nxt
+        # drop repeated and collinear vertices
+        changed = True
+        while changed and len(poly) >= 3:
+            changed = False
+            for k in range(len(poly)):
+                a, b, c2 = poly[k - 1], poly[k], poly[(k + 1) % len(poly)]
+                cr = (b[0] - a[0]) * (c2[1] - b[1]) - (b[1] - a[1]) * (c2[0] - b[0])
+                if b == a or cr == 0:
+                    del poly[k]
+                    changed = True
+                    break
+        if len(poly) < 3:
+            poly = []
+        out.append(np.array([[float(x), float(y)] for x, y in poly]).reshape(-1, 2))
+    return out
+
+
 def lloyd(mesh, X, w):
     """Cell masses and centroids (``include/MA/lloyd.hpp:126-144``)."""
     m, mom = first_moment(mesh, X, w)

@@ -229,3 +229,79 @@ def test_set_mesh_image_matches_host_mesh(engine, shape):
     assert abs(f1 - f2) <= 1e-14 * abs(f2)
     assert np.abs(g1 - g2).max() <= 1e-14 * np.abs(g2).max()
     assert abs(H1 - H2).max() <= 1e-14 * abs(H2).max()
+
+
+def _poly_area(q):
+    return 0.5 * np.sum(q[:, 0] * np.roll(q[:, 1], -1) - np.roll(q[:, 0], -1) * q[:, 1])
+
+
+def _convex_polygon(seed):
+    from scipy.spatial import ConvexHull
+
+    pts = np.random.default_rng(seed).uniform(-1, 1, (12, 2))
+    return pts[ConvexHull(pts).vertices]
+
+
+@pytest.mark.parametrize("path", list(PATHS))
+@pytest.mark.parametrize("shape", ["square", "hexagon", "random"])
+def test_cell_polygons_vs_oracle(engine, shape, path):
+    if shape == "square":
+        P = np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]])
+    elif shape == "hexagon":
+        t = np.linspace(0, 2 * np.pi, 7)[:-1]
+        P = np.stack([np.cos(t), np.sin(t)], 1)[::-1]  # clockwise input
+    else:
+        P = _convex_polygon(11)
+    rng = np.random.default_rng(len(shape))
+    N = 60
+    X = rng.uniform(-0.8, 0.8, (N, 2))
+    w = rng.uniform(-0.02, 0.02, N)
+    print(f"polygon {shape} path {path}", flush=True)
+    engine.set_option("force_general", PATHS[path])
+    try:
+        engine.set_mesh_polygon(P).set_diracs(X)
+        start, xy = engine.cell_polygons(w)
+    finally:
+        engine.set_option("force_general", 0)
+    ref = oracle.voronoi_polygon_intersection(P, X, w)
+    for i in range(N):
+        q = xy[start[i]:start[i + 1]]
+        r = ref[i]
+        assert len(q) == len(r), (i, q, r)
+        if len(r):
+            d = np.abs(q[:, None, :] - r[None, :, :]).max(axis=2)
+            assert d.min(axis=1).max() <= 1e-12, (i, q, r)
+            assert _poly_area(q) > 0
+    total = sum(_poly_area(xy[start[i]:start[i + 1]]) for i in range(N) if start[i + 1] > start[i])
+    assert np.isclose(total, abs(_poly_area(P)), rtol=1e-13)
+
+
+def test_cell_polygons_lattice_and_large(engine):
+    from mongeampere_amd.api import voronoi_polygon_intersection
+
+    SQ = np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]])
+    k = 7
+    c = -1 + (2 * np.arange(k) + 1) / k
+    X = np.stack(np.meshgrid(c, c, indexing="ij"), -1).reshape(-1, 2)
+    cells = voronoi_polygon_intersection(SQ, X, np.zeros(k * k), engine=engine)
+    for i, q in enumerate(cells):
+        assert q.shape == (4, 2)
+        assert abs(_poly_area(q) - (2.0 / k) ** 2) <= 1e-14
+        assert np.abs(q.mean(axis=0) - X[i]).max() <= 1e-14
+    # the reference's test_voronoi / test_power workload at N = 100000:
+    # random Diracs on [-1,1]^2, the cells tile the square (area 4)
+    X = np.random.default_rng(3).uniform(-1, 1, (100000, 2))
+    engine.set_mesh_polygon(SQ).set_diracs(X)
+    start, xy = engine.cell_polygons(np.zeros(100000))
+    cnt = np.diff(start)
+    assert cnt.min() >= 3
+    idx = np.repeat(np.arange(100000), cnt)
+    nxt = np.arange(xy.shape[0]) + 1
+    nxt[start[1:] - 1] = start[:-1]
+    cross = xy[:, 0] * xy[nxt, 1] - xy[nxt, 0] * xy[:, 1]
+    areas = 0.5 * np.bincount(idx, cross, minlength=100000)
+    assert areas.min() > 0
+    assert abs(areas.sum() - 4.0) <= 1e-11
+    m, _ = engine.first_moment(np.zeros(100000))
+    # (shoelace areas in absolute coordinates carry ~1e-16 of rounding)
+    assert np.abs(areas - m).max() <= 1e-11 * m.max()

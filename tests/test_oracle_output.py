@@ -128,3 +128,40 @@ def test_image_mesh_constant_and_linear():
     hi = mesh.vval > 0.5
     assert np.all(mesh.vxy[hi, 1] == 1.0) and hi.sum() == W
     assert mesh.vxy[1, 0] == -1.0 and mesh.vxy[1, 1] == -1.0 + 2.0 / (H - 1)
+
+
+def _poly_area(q):
+    return 0.5 * np.sum(q[:, 0] * np.roll(q[:, 1], -1) - np.roll(q[:, 0], -1) * q[:, 1])
+
+
+SQUARE = np.array([[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0]])
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 5])
+def test_polygon_cells_lattice_closed_form(k):
+    # centred k x k lattice on [-1,1]^2, w = 0: square cells of side 2/k
+    c = -1 + (2 * np.arange(k) + 1) / k
+    X = np.stack(np.meshgrid(c, c, indexing="ij"), -1).reshape(-1, 2)
+    cells = oracle.voronoi_polygon_intersection(SQUARE, X, np.zeros(k * k))
+    for i, q in enumerate(cells):
+        assert q.shape == (4, 2)
+        assert np.isclose(_poly_area(q), (2.0 / k) ** 2, rtol=1e-15)
+        assert np.allclose(q.mean(axis=0), X[i], atol=1e-15)
+
+
+def test_polygon_cells_two_diracs_and_total_area():
+    # two Diracs on the x axis, weights (w0, w1): bisector at
+    # x = (|y1|^2 - |y0|^2 - w1 + w0) / (2 (x1 - x0))
+    X = np.array([[-0.5, 0.0], [0.5, 0.0]])
+    w = np.array([0.1, -0.1])
+    cells = oracle.voronoi_polygon_intersection(SQUARE, X, w)
+    xb = (0.25 - 0.25 + 0.1 + 0.1) / 2.0
+    assert np.isclose(_poly_area(cells[0]), 2 * (xb + 1), rtol=1e-15)
+    assert np.isclose(_poly_area(cells[1]), 2 * (1 - xb), rtol=1e-15)
+    # random Diracs on a hexagon: the cells tile it (tests/test_voronoi.cpp:50)
+    t = np.linspace(0, 2 * np.pi, 7)[:-1]
+    hexa = np.stack([np.cos(t), np.sin(t)], 1)
+    rng = np.random.default_rng(1)
+    X = rng.uniform(-0.7, 0.7, (40, 2))
+    cells = oracle.voronoi_polygon_intersection(hexa, X, rng.uniform(-0.01, 0.01, 40))
+    assert np.isclose(sum(_poly_area(q) for q in cells if len(q)), _poly_area(hexa), rtol=1e-13)
