@@ -66,6 +66,15 @@ def cpu_baseline_sample(p, frac=0.5):
     oracle.kantorovich(mesh, X, w)
     dt = time.perf_counter() - t0
     per_dirac = dt / X.shape[0]
+    cpu = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {
         "value": 1.0 / (per_dirac * p.X.shape[0]),
         "unit": "evals/s",
@@ -73,7 +82,8 @@ def cpu_baseline_sample(p, frac=0.5):
         "kind": "port",
         "sample": (f"FP64 oracle (C restatement, exact predicates) on the window [0,{frac}]^2 of "
                    f"the same workload: {X.shape[0]} Diracs, {tri.shape[0]} triangles, "
-                   f"{dt:.2f} s; scaled by Dirac count to N={p.X.shape[0]}"),
+                   f"{dt:.2f} s; scaled by Dirac count to N={p.X.shape[0]}; one thread of "
+                   f"{cpu} ({os.cpu_count()} logical CPUs visible)"),
     }
 
 
@@ -260,6 +270,12 @@ def main():
         for _ in range(nh):
             e.kantorovich(w_host, prec, hessian=False)
         out[f"evals_per_s_{prec}_w0_host_buffers"] = nh / (time.perf_counter() - t0)
+        # accuracy of the mixed evaluation against the FP64 one (masses)
+        _, g_m, _ = e.kantorovich(w_host, "mixed", hessian=False)
+        _, g_d, _ = e.kantorovich(w_host, "fp64", hessian=False)
+        out["mixed_vs_fp64_mass_w0"] = {"max_rel": float(np.abs(g_m - g_d).max() / g_d.mean()),
+                                        "mean_rel": float(np.abs(g_m - g_d).mean() / g_d.mean()),
+                                        "unit": "of the mean cell mass"}
         # MA::second_moment (FP64; masses, first and second moments to the host)
         e.second_moment(w_host)
         t0 = time.perf_counter()

@@ -258,7 +258,13 @@ ma_status ma_cell_paths(ma_engine *e, int64_t count, const int64_t *ids, int32_t
  * paths); MA_OPT_CAP_CAND: initial capacity of the per-bin candidate
  * lists (grows automatically on overflow); MA_OPT_GRAPHS (0/1, default 1):
  * replay batches of conjugate-gradient iterations from a HIP graph. */
-typedef enum { MA_OPT_FORCE_GENERAL = 1, MA_OPT_CAP_CAND = 2, MA_OPT_GRAPHS = 3 } ma_option;
+typedef enum {
+  MA_OPT_FORCE_GENERAL = 1,
+  MA_OPT_CAP_CAND = 2,
+  MA_OPT_GRAPHS = 3,
+  MA_OPT_LOCAL_RINGS = 4, /* rings of bins searched around each Dirac (0..64, default 8) */
+  MA_OPT_WALK_BUDGET = 5  /* pyramid nodes per cell before the bin path (default 512) */
+} ma_option;
 ma_status ma_set_option(ma_engine *e, int option, int64_t value);
 void ma_default_solve_opts(ma_solve_opts *o);
 

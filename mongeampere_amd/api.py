@@ -103,7 +103,8 @@ class Engine:
             raise EngineError(st, msg.decode() if msg else "")
 
     def set_option(self, name, value):
-        key = {"force_general": 1, "cap_cand": 2, "graphs": 3}[name]
+        key = {"force_general": 1, "cap_cand": 2, "graphs": 3, "local_rings": 4,
+               "walk_budget": 5}[name]
         self._check(self._L.ma_set_option(self._h, key, int(value)))
         return self
 

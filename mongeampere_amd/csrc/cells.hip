@@ -31,8 +31,6 @@ namespace ma {
 
 constexpr int kMaxFast = 16;
 constexpr int kCellBlock = 128;
-constexpr int kLocalRings = 8;  // site-centric search radius, in bins
-constexpr int kWalkBudget = 512;  // pyramid nodes a cell may visit before the bin path
 constexpr int kNodeDiracs = 8;    // pyramid nodes with at most this many Diracs are taken whole
 constexpr double kBoxBins = 8.0;  // FP32 starting box half-width, in bins
 constexpr int kBoxLabel = -3;     // edge label of that box's cut
@@ -82,6 +80,7 @@ struct CellArgs {
   int32_t *flags;
   unsigned long long *counters;
   int force_general;  // route every cell to the general kernel (testing)
+  int walk_budget;    // pyramid nodes a cell may visit before the bin path
   FragOut fo;         // fragment polygons (mode 1, when fo.count is set)
 };
 
@@ -467,7 +466,7 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
     uint32_t m = 0;
     int visits = 0;
     while (n > 0 && !fail) {
-      if (++visits > kWalkBudget) {
+      if (++visits > a.walk_budget) {
         // weights with a large spread leave many nodes unprunable: the cell
         // is rebuilt from the per-bin candidate owners instead
         over_budget = true;
@@ -1305,6 +1304,7 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   a.flags = B.flags.p;
   a.counters = B.counters.p;
   a.force_general = e->force_general;
+  a.walk_budget = e->walk_budget;
   a.bin_start = e->bin_start.p;
   a.triv = e->triv.p;
   a.dval = B.dval.p;
@@ -1316,10 +1316,10 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   for (int l = 0; l <= e->grid.L; l++) po.o[l] = e->pyr_off[l];
   if (fp32)
     hipLaunchKernelGGL(k_cell_local<float>, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0,
-                       e->stream, a, B.pyr.p, B.pyr3.p, po, kLocalRings);
+                       e->stream, a, B.pyr.p, B.pyr3.p, po, e->local_rings);
   else
     hipLaunchKernelGGL(k_cell_local<double>, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0,
-                       e->stream, a, B.pyr.p, B.pyr3.p, po, kLocalRings);
+                       e->stream, a, B.pyr.p, B.pyr3.p, po, e->local_rings);
   {
     int32_t fl0[4];
     unsigned long long ct0[8];

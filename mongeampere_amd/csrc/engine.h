@@ -90,6 +90,11 @@ constexpr int kBigRect = 64;
 // one-workgroup-per-cell kernel.
 constexpr int kHugeRoute = 32768;
 constexpr int kLongRows = 1024;  // pool of staged rows longer than kMaxRow
+// Site-centric cell search: rings of bins visited before the pyramid walk,
+// and pyramid nodes a cell may visit before it is left to the bin path
+// (defaults; MA_OPT_LOCAL_RINGS / MA_OPT_WALK_BUDGET).
+constexpr int kLocalRings = 8;
+constexpr int kWalkBudget = 512;
 
 // Fragment polygons written out by the first-moment kernels while `count` is
 // set (ma_fragments).  Each fragment of positive area reserves one record and
@@ -199,6 +204,8 @@ struct ma_engine {
   int cap_cand = 48;
   int force_general = 0;  // MA_OPT_FORCE_GENERAL
   int use_graphs = 1;     // MA_OPT_GRAPHS: replay CG batches from a HIP graph
+  int local_rings = ma::kLocalRings;  // MA_OPT_LOCAL_RINGS
+  int walk_budget = ma::kWalkBudget;  // MA_OPT_WALK_BUDGET
   bool ran_candidates = false;  // the last evaluation built per-bin candidate lists
   int cap_row = ma::kMaxRow;
 

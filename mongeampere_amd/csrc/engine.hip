@@ -974,6 +974,14 @@ ma_status ma_set_option(ma_engine *e, int option, int64_t value) {
     case MA_OPT_GRAPHS:
       e->use_graphs = value != 0;
       return MA_OK;
+    case MA_OPT_LOCAL_RINGS:
+      if (value < 0 || value > 64) return MA_BAD_INPUT;
+      e->local_rings = (int)value;
+      return MA_OK;
+    case MA_OPT_WALK_BUDGET:
+      if (value < 0 || value > (1 << 20)) return MA_BAD_INPUT;
+      e->walk_budget = (int)value;
+      return MA_OK;
     default:
       return MA_BAD_INPUT;
   }

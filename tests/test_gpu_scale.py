@@ -191,6 +191,25 @@ def test_c5_mixed_forward_and_adjoint(engine, c5):
     # adjoint: Phi is linear in rho, so <rho, dPhi/drho> = Phi = -fval
     dY, dval = engine.adjoint(w)
     assert abs(p.mesh.vval @ dval + f64) <= 1e-10 * abs(f64)
+    # device memory of a fresh engine through both evaluations and the adjoint
+    import ctypes
+
+    from mongeampere_amd.api import Engine
+
+    hip = ctypes.CDLL("libamdhip64.so.7")
+    free0, tot = ctypes.c_size_t(), ctypes.c_size_t()
+    assert hip.hipMemGetInfo(ctypes.byref(free0), ctypes.byref(tot)) == 0
+    with Engine(0) as e2:
+        e2.set_mesh(p.mesh).set_diracs(p.X)
+        e2.kantorovich(w, precision="fp64")
+        e2.kantorovich(w, precision="mixed")
+        e2.adjoint(w)
+        free1 = ctypes.c_size_t()
+        assert hip.hipMemGetInfo(ctypes.byref(free1), ctypes.byref(tot)) == 0
+    used = (free0.value - free1.value) / 2**30
+    print(f"C5 device memory of one engine (evaluations + adjoint): {used:.2f} GiB "
+          f"of {tot.value / 2**30:.0f} GiB", flush=True)
+    assert used < 0.25 * tot.value / 2**30
     # position part at w = 0 against the oracle on a window of the annulus
     sel = (p.X[:, 0] > 0.5) & (p.X[:, 0] < 0.52) & (p.X[:, 1] > -0.01) & (p.X[:, 1] < 0.01)
     m, cen = engine.lloyd(w)
