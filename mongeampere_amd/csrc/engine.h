@@ -103,7 +103,7 @@ constexpr int kWalkBudget = 512;
 struct FragOut {
   unsigned long long *count = nullptr;  // [0] fragments, [1] vertices
   int32_t *cell = nullptr;  // caller index of the Dirac
-  int32_t *tri = nullptr;   // triangle index (caller order)
+  int32_t *tri = nullptr;   // triangle index (engine order, see ma_engine::tperm)
   int32_t *nv = nullptr;    // vertex count
   int64_t *start = nullptr; // first vertex
   double *xy = nullptr;     // absolute coordinates, interleaved, counter-clockwise
@@ -183,9 +183,13 @@ struct ma_engine {
   // mesh
   int64_t nv = 0, nt = 0;
   double dom[4] = {0, 0, 0, 0};  // domain bounding box
+  // triangles in Morton order of their centroids (tperm: that order -> the
+  // caller's triangle index), so that the triangles a wave of pairs reads
+  // share cache lines
   ma::DBuf<ma::TriData> tri;
   ma::DBuf<int32_t> triv;  // ccw vertex ids (nt x 3)
   ma::DBuf<float4> tbox;   // FP32 bounding boxes, rounded outwards
+  ma::DBuf<int32_t> tperm;
   ma::DBuf<double> vxy, vval;
   bool have_mesh = false;
 
@@ -231,6 +235,10 @@ ma_status need(ma_engine *e);
 // domain box `box` (x0, y0, x1, y1).
 ma_status finish_mesh(ma_engine *e, int64_t nv, int64_t nt, const int32_t *d_tri,
                       const double box[4]);
+
+// Reorders e->tri / triv / tbox (caller order on entry) by the Morton code of
+// the triangle centroids and sets e->tperm (output.hip).
+ma_status sort_triangles(ma_engine *e, int64_t nt);
 
 // Geometry (geometry.hip)
 hipError_t build_dirac_grid(ma_engine *e, const double *d_X, int layout);

@@ -584,6 +584,8 @@ ma_status ma::finish_mesh(ma_engine *e, int64_t nv, int64_t nt, const int32_t *d
     e->err = "degenerate (zero-area) triangle";
     return MA_BAD_INPUT;
   }
+  ma_status st = sort_triangles(e, nt);
+  if (st != MA_OK) return st;
   e->nv = nv;
   e->nt = nt;
   e->have_mesh = true;

@@ -22,11 +22,12 @@ namespace ma {
 // Fragment polygons
 // ---------------------------------------------------------------------------
 
-__global__ void k_frag_keys(int64_t nf, const int32_t *cell, const int32_t *tri, uint64_t *key,
-                            int32_t *idx) {
+// Sort key (Dirac, caller's triangle index) of every fragment record.
+__global__ void k_frag_keys(int64_t nf, const int32_t *cell, const int32_t *tri,
+                            const int32_t *tperm, uint64_t *key, int32_t *idx) {
   const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  key[f] = ((uint64_t)(uint32_t)cell[f] << 32) | (uint32_t)tri[f];
+  key[f] = ((uint64_t)(uint32_t)cell[f] << 32) | (uint32_t)tperm[tri[f]];
   idx[f] = (int32_t)f;
 }
 
@@ -115,7 +116,7 @@ static ma_status sort_fragments(ma_engine *e, int64_t nf, int64_t nvert) {
     return MA_OK;
   }
   hipLaunchKernelGGL(k_frag_keys, dim3(nblocks(nf)), dim3(kBlock), 0, e->stream, nf, O.fcell.p,
-                     O.ftri.p, O.key.p, O.idx.p);
+                     O.ftri.p, e->tperm.p, O.key.p, O.idx.p);
   // key bits: Dirac index above bit 32, triangle index below
   int hi = 33;
   while (hi < 64 && ((int64_t)1 << (hi - 32)) <= e->n) hi++;
@@ -137,6 +138,67 @@ static ma_status sort_fragments(ma_engine *e, int64_t nf, int64_t nvert) {
                      O.key2.p, O.fstart.p, O.fnv.p, O.fxy.p, O.sstart.p, O.scell.p, O.stri.p,
                      O.sxy.p);
   MA_HIP_TRY(hipGetLastError());
+  return MA_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Triangle order
+// ---------------------------------------------------------------------------
+
+// Morton code (16 bits per axis) of each triangle's centroid in the domain box.
+__global__ void k_tri_keys(const TriData *tri, int64_t nt, double x0, double y0, double sx,
+                           double sy, uint32_t *key, int32_t *idx) {
+  const int64_t f = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (f >= nt) return;
+  const TriData t = tri[f];
+  const double cx = (t.x[0] + t.x[1] + t.x[2]) * (1.0 / 3.0);
+  const double cy = (t.y[0] + t.y[1] + t.y[2]) * (1.0 / 3.0);
+  const uint32_t ix = (uint32_t)fmin(fmax((cx - x0) * sx, 0.0), 65535.0);
+  const uint32_t iy = (uint32_t)fmin(fmax((cy - y0) * sy, 0.0), 65535.0);
+  key[f] = (uint32_t)morton2(ix, iy);
+  idx[f] = (int32_t)f;
+}
+
+__global__ void k_tri_permute(const int32_t *src, int64_t nt, const TriData *tri_in,
+                              const int32_t *triv_in, const float4 *tbox_in, TriData *tri_out,
+                              int32_t *triv_out, float4 *tbox_out) {
+  const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (s >= nt) return;
+  const int64_t f = src[s];
+  tri_out[s] = tri_in[f];
+  tbox_out[s] = tbox_in[f];
+  for (int c = 0; c < 3; c++) triv_out[3 * s + c] = triv_in[3 * f + c];
+}
+
+ma_status sort_triangles(ma_engine *e, int64_t nt) {
+  DBuf<uint32_t> key, key2;
+  DBuf<int32_t> idx, triv2;
+  DBuf<TriData> tri2;
+  DBuf<float4> tbox2;
+  MA_HIP_TRY(key.ensure((size_t)nt));
+  MA_HIP_TRY(key2.ensure((size_t)nt));
+  MA_HIP_TRY(idx.ensure((size_t)nt));
+  MA_HIP_TRY(e->tperm.ensure((size_t)nt));
+  const double ext = std::max(e->dom[2] - e->dom[0], e->dom[3] - e->dom[1]);
+  const double sc = ext > 0 ? 65535.0 / ext : 0.0;
+  hipLaunchKernelGGL(k_tri_keys, dim3(nblocks(nt)), dim3(kBlock), 0, e->stream, e->tri.p, nt,
+                     e->dom[0], e->dom[1], sc, sc, key.p, idx.p);
+  size_t tmp_bytes = 0;
+  MA_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key.p, key2.p, idx.p,
+                                                e->tperm.p, (int)nt, 0, 32, e->stream));
+  MA_HIP_TRY(e->ob.sort_tmp.ensure(tmp_bytes));
+  MA_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(e->ob.sort_tmp.p, tmp_bytes, key.p, key2.p, idx.p,
+                                                e->tperm.p, (int)nt, 0, 32, e->stream));
+  MA_HIP_TRY(tri2.ensure((size_t)nt));
+  MA_HIP_TRY(triv2.ensure((size_t)(3 * nt)));
+  MA_HIP_TRY(tbox2.ensure((size_t)nt));
+  hipLaunchKernelGGL(k_tri_permute, dim3(nblocks(nt)), dim3(kBlock), 0, e->stream, e->tperm.p, nt,
+                     e->tri.p, e->triv.p, e->tbox.p, tri2.p, triv2.p, tbox2.p);
+  MA_HIP_TRY(hipGetLastError());
+  MA_HIP_TRY(hipStreamSynchronize(e->stream));
+  e->tri.swap(tri2);
+  e->triv.swap(triv2);
+  e->tbox.swap(tbox2);
   return MA_OK;
 }
 
