@@ -896,13 +896,26 @@ __device__ inline void frag_setup(const CellArgs &a, int64_t k, const TriData &t
   }
 }
 
+// Vertex v of the cell polygon of k (label: v, or -1 on a domain edge) from
+// the evaluation's polygon arrays.
+template <class R>
+struct PolyGlobal {
+  const CellArgs &a;
+  int64_t k;
+  __device__ void operator()(int v, SV<R> &c) const {
+    const int64_t N = a.n;
+    c.x = CellBuf<R>::px(a)[v * N + k];
+    c.y = CellBuf<R>::py(a)[v * N + k];
+    c.lab = (a.plab[v * N + k] >= 0) ? v : -1;
+  }
+};
+
 // Streams the n-vertex cell polygon of k through the three clip stages;
-// sink(v) receives the fragment's vertices in order.
-template <class R, class Sink>
-__device__ inline void frag_clip(const CellArgs &a, int64_t k, int n, Stage<R> &s1,
-                                 Stage<R> &s2, Stage<R> &s3, Sink &&sink) {
-  const int64_t N = a.n;
-  const R *px = CellBuf<R>::px(a), *py = CellBuf<R>::py(a);
+// src(v, c) fetches vertex v, sink(v) receives the fragment's vertices in
+// order.
+template <class R, class Src, class Sink>
+__device__ inline void frag_clip(const Src &src, int n, Stage<R> &s1, Stage<R> &s2, Stage<R> &s3,
+                                 Sink &&sink) {
   SV<R> o0, o1, q0, q1, r0, r1;
   auto to3 = [&](const SV<R> &v) {
     const int m3 = s3.push(v, r0, r1);
@@ -916,9 +929,7 @@ __device__ inline void frag_clip(const CellArgs &a, int64_t k, int n, Stage<R> &
   };
   for (int v = 0; v < n; v++) {
     SV<R> c;
-    c.x = px[v * N + k];
-    c.y = py[v * N + k];
-    c.lab = (a.plab[v * N + k] >= 0) ? v : -1;
+    src(v, c);
     const int m1 = s1.push(c, o0, o1);
     if (m1 > 0) to2(o0);
     if (m1 > 1) to2(o1);
@@ -934,10 +945,15 @@ __device__ inline void frag_clip(const CellArgs &a, int64_t k, int n, Stage<R> &
   if (m3 > 1) sink(r1);
 }
 
-// Fragment vertices staged per thread in LDS by k_frag (FP32 / FP64).
+// Fragment vertices staged per thread in LDS by k_frag (FP32 / FP64), and
+// the number of cells whose polygons a block of k_frag holds in LDS.
 template <class R>
 struct FragCap {
-  static constexpr int value = sizeof(R) == sizeof(float) ? 9 : 10;
+  static constexpr int value = sizeof(R) == sizeof(float) ? 9 : 8;
+};
+template <class R>
+struct FragCells {
+  static constexpr int value = sizeof(R) == sizeof(float) ? 48 : 32;
 };
 
 template <int MODE, class R>
@@ -952,6 +968,36 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   const int64_t k = pair_cell[p];
   const TriData t = a.tri[pair_tri[p]];
   const int n = valid ? a.pn[k] : 0;
+  // The block's pairs belong to a contiguous range of cells (pairs come
+  // sorted by cell): the polygons of its first kCells cells are loaded into
+  // LDS at once with coalesced loads, instead of one dependent global load
+  // per vertex inside the clip loop.
+  constexpr int kCells = FragCells<R>::value;
+  __shared__ R cpx[kMaxFast][kCells], cpy[kMaxFast][kCells];
+  __shared__ unsigned char clab[kMaxFast][kCells];
+  __shared__ int cpn[kCells];
+  const int tt = (int)threadIdx.x;
+  const int64_t pb = blockIdx.x * (int64_t)blockDim.x;
+  const int64_t kb0 = pair_cell[pb];
+  const int nc = (int)min((int64_t)pair_cell[min(pb + (int64_t)blockDim.x, npairs) - 1] - kb0 + 1,
+                          (int64_t)kCells);
+  if (tt < nc) cpn[tt] = max(a.pn[kb0 + tt], 0);
+  __syncthreads();
+  {
+    const int64_t N = a.n;
+    const R *gx = CellBuf<R>::px(a), *gy = CellBuf<R>::py(a);
+    for (int idx = tt; idx < kMaxFast * nc; idx += blockDim.x) {
+      const int v = idx / nc, c = idx - v * nc;
+      if (v < cpn[c]) {
+        const int64_t kk = kb0 + c;
+        cpx[v][c] = gx[v * N + kk];
+        cpy[v][c] = gy[v * N + kk];
+        clab[v][c] = a.plab[v * N + kk] >= 0;
+      }
+    }
+  }
+  __syncthreads();
+  const int ci = (int)(k - kb0);
   Stage<R> s1, s2, s3;
   Integrator<MODE, R> I;
   frag_setup<MODE, R>(a, k, t, s1, s2, s3, I);
@@ -967,9 +1013,17 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   R *ox = reinterpret_cast<R *>(lds);
   R *oy = ox + kCap * 256;
   int *ol = reinterpret_cast<int *>(oy + kCap * 256);
-  const int tt = (int)threadIdx.x;
   int nout = 0;
-  frag_clip<R>(a, k, n, s1, s2, s3, [&](const SV<R> &v) {
+  auto src = [&](int v, SV<R> &c) {
+    if (ci < kCells) {
+      c.x = cpx[v][ci];
+      c.y = cpy[v][ci];
+      c.lab = clab[v][ci] ? v : -1;
+    } else {
+      PolyGlobal<R>{a, k}(v, c);
+    }
+  };
+  frag_clip<R>(src, n, s1, s2, s3, [&](const SV<R> &v) {
     if (nout < kCap) {
       ox[nout * 256 + tt] = v.x;
       oy[nout * 256 + tt] = v.y;
@@ -1036,7 +1090,16 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
     skey[t] = counted ? (int)k : (valid ? -2 - (int)(k & 0x3fffffff) : -1);
 #pragma unroll
     for (int c = 0; c < kQ; c++) sval[c][t] = counted ? q[c] : 0.0;
+    // fragment count: per wave into LDS, one global atomic per block (a
+    // same-address atomic per wave serialises across the whole grid)
+    __shared__ int wcnt[4];
+    const unsigned long long bal = __ballot(counted);
+    if ((t & 63) == 0) wcnt[t >> 6] = __popcll(bal);
     __syncthreads();
+    if (t == 0) {
+      const int c4 = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      if (c4) atomicAdd(&a.counters[0], (unsigned long long)c4);
+    }
     // runs are keyed by cell; uncounted fragments keep their cell's run
     // boundaries (key -2 - k) but contribute nothing
     const int mykey = valid ? (int)k : -1;
@@ -1072,7 +1135,6 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
       }
     }
   }
-  wave_count(&a.counters[0], counted);
 }
 
 // The pairs whose fragment outgrew k_frag's staging: streaming integration,
@@ -1088,7 +1150,7 @@ __global__ void __launch_bounds__(256) k_frag_wide(CellArgs a, const int32_t *__
     Stage<R> s1, s2, s3;
     Integrator<MODE, R> I;
     frag_setup<MODE, R>(a, k, t, s1, s2, s3, I);
-    frag_clip<R>(a, k, a.pn[k], s1, s2, s3, [&](const SV<R> &v) { I.push(v); });
+    frag_clip<R>(PolyGlobal<R>{a, k}, a.pn[k], s1, s2, s3, [&](const SV<R> &v) { I.push(v); });
     I.finish();
     if (!(I.cnt >= 3 && I.positive())) continue;
     if (MODE == 1 && a.fo.count != nullptr) {
@@ -1097,7 +1159,7 @@ __global__ void __launch_bounds__(256) k_frag_wide(CellArgs a, const int32_t *__
       DistinctRun d(eps);
       Integrator<MODE, R> J;  // (I still holds this pair's sums)
       frag_setup<MODE, R>(a, k, t, s1, s2, s3, J);
-      frag_clip<R>(a, k, a.pn[k], s1, s2, s3,
+      frag_clip<R>(PolyGlobal<R>{a, k}, a.pn[k], s1, s2, s3,
                    [&](const SV<R> &q) { d.take((double)q.x, (double)q.y); });
       const int m = d.count();
       unsigned long long f, v;
@@ -1110,7 +1172,7 @@ __global__ void __launch_bounds__(256) k_frag_wide(CellArgs a, const int32_t *__
         const double ox = a.pos[2 * k], oy = a.pos[2 * k + 1];
         DistinctRun dw(eps);
         frag_setup<MODE, R>(a, k, t, s1, s2, s3, J);
-        frag_clip<R>(a, k, a.pn[k], s1, s2, s3, [&](const SV<R> &q) {
+        frag_clip<R>(PolyGlobal<R>{a, k}, a.pn[k], s1, s2, s3, [&](const SV<R> &q) {
           const int at = dw.cnt;
           if (dw.take((double)q.x, (double)q.y) && at < m) {
             a.fo.xy[2 * (v + at)] = (double)q.x + ox;
