@@ -81,6 +81,7 @@ struct CellArgs {
   unsigned long long *counters;
   int force_general;  // route every cell to the general kernel (testing)
   int walk_budget;    // pyramid nodes a cell may visit before the bin path
+  unsigned long long *stripes;  // striped event counters (EvalBuffers::stripes)
   FragOut fo;         // fragment polygons (mode 1, when fo.count is set)
 };
 
@@ -518,8 +519,11 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
     else
       fail = true;
   }
-  wave_count(&a.flags[2], walk);  // cells that needed the pyramid walk
-  wave_count(&a.counters[6], over_budget && !fail);  // cells left to the bin path
+  {
+    const int lane16 = (int)((k >> 6) & 15);
+    wave_count(&a.stripes[lane16 * 16], walk);  // cells that needed the pyramid walk
+    wave_count(&a.stripes[(16 + lane16) * 16], over_budget && !fail);  // left to the bin path
+  }
   if (Tol<R>::redo > R(0) && nredo > 0) atomicAdd(&a.counters[5], (unsigned long long)nredo);
   if (fail) {
     a.pn[k] = -1;
@@ -945,15 +949,10 @@ __device__ inline void frag_clip(const Src &src, int n, Stage<R> &s1, Stage<R> &
   if (m3 > 1) sink(r1);
 }
 
-// Fragment vertices staged per thread in LDS by k_frag (FP32 / FP64), and
-// the number of cells whose polygons a block of k_frag holds in LDS.
+// Fragment vertices staged per thread in LDS by k_frag (FP32 / FP64).
 template <class R>
 struct FragCap {
   static constexpr int value = sizeof(R) == sizeof(float) ? 9 : 8;
-};
-template <class R>
-struct FragCells {
-  static constexpr int value = sizeof(R) == sizeof(float) ? 48 : 32;
 };
 
 template <int MODE, class R>
@@ -968,36 +967,6 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   const int64_t k = pair_cell[p];
   const TriData t = a.tri[pair_tri[p]];
   const int n = valid ? a.pn[k] : 0;
-  // The block's pairs belong to a contiguous range of cells (pairs come
-  // sorted by cell): the polygons of its first kCells cells are loaded into
-  // LDS at once with coalesced loads, instead of one dependent global load
-  // per vertex inside the clip loop.
-  constexpr int kCells = FragCells<R>::value;
-  __shared__ R cpx[kMaxFast][kCells], cpy[kMaxFast][kCells];
-  __shared__ unsigned char clab[kMaxFast][kCells];
-  __shared__ int cpn[kCells];
-  const int tt = (int)threadIdx.x;
-  const int64_t pb = blockIdx.x * (int64_t)blockDim.x;
-  const int64_t kb0 = pair_cell[pb];
-  const int nc = (int)min((int64_t)pair_cell[min(pb + (int64_t)blockDim.x, npairs) - 1] - kb0 + 1,
-                          (int64_t)kCells);
-  if (tt < nc) cpn[tt] = max(a.pn[kb0 + tt], 0);
-  __syncthreads();
-  {
-    const int64_t N = a.n;
-    const R *gx = CellBuf<R>::px(a), *gy = CellBuf<R>::py(a);
-    for (int idx = tt; idx < kMaxFast * nc; idx += blockDim.x) {
-      const int v = idx / nc, c = idx - v * nc;
-      if (v < cpn[c]) {
-        const int64_t kk = kb0 + c;
-        cpx[v][c] = gx[v * N + kk];
-        cpy[v][c] = gy[v * N + kk];
-        clab[v][c] = a.plab[v * N + kk] >= 0;
-      }
-    }
-  }
-  __syncthreads();
-  const int ci = (int)(k - kb0);
   Stage<R> s1, s2, s3;
   Integrator<MODE, R> I;
   frag_setup<MODE, R>(a, k, t, s1, s2, s3, I);
@@ -1014,16 +983,8 @@ __global__ void __launch_bounds__(256) k_frag(CellArgs a, int64_t npairs,
   R *oy = ox + kCap * 256;
   int *ol = reinterpret_cast<int *>(oy + kCap * 256);
   int nout = 0;
-  auto src = [&](int v, SV<R> &c) {
-    if (ci < kCells) {
-      c.x = cpx[v][ci];
-      c.y = cpy[v][ci];
-      c.lab = clab[v][ci] ? v : -1;
-    } else {
-      PolyGlobal<R>{a, k}(v, c);
-    }
-  };
-  frag_clip<R>(src, n, s1, s2, s3, [&](const SV<R> &v) {
+  const int tt = (int)threadIdx.x;
+  frag_clip<R>(PolyGlobal<R>{a, k}, n, s1, s2, s3, [&](const SV<R> &v) {
     if (nout < kCap) {
       ox[nout * 256 + tt] = v.x;
       oy[nout * 256 + tt] = v.y;
@@ -1367,6 +1328,9 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   a.counters = B.counters.p;
   a.force_general = e->force_general;
   a.walk_budget = e->walk_budget;
+  MA_HIP_TRY(B.stripes.ensure(32 * 16));
+  MA_HIP_TRY(hipMemsetAsync(B.stripes.p, 0, sizeof(unsigned long long) * 32 * 16, e->stream));
+  a.stripes = B.stripes.p;
   a.bin_start = e->bin_start.p;
   a.triv = e->triv.p;
   a.dval = B.dval.p;
@@ -1384,11 +1348,18 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
                        e->stream, a, B.pyr.p, B.pyr3.p, po, e->local_rings);
   {
     int32_t fl0[4];
-    unsigned long long ct0[8];
+    unsigned long long ct0[8], st[32 * 16];
     MA_HIP_TRY(hipMemcpyAsync(fl0, B.flags.p, sizeof fl0, hipMemcpyDeviceToHost, e->stream));
     MA_HIP_TRY(hipMemcpyAsync(ct0, B.counters.p, sizeof ct0, hipMemcpyDeviceToHost, e->stream));
+    MA_HIP_TRY(hipMemcpyAsync(st, B.stripes.p, sizeof st, hipMemcpyDeviceToHost, e->stream));
     MA_HIP_TRY(hipStreamSynchronize(e->stream));
-    e->info.nbinpath = fl0[2];
+    unsigned long long nwalk = 0, nbins = 0;
+    for (int i = 0; i < 16; i++) {
+      nwalk += st[i * 16];
+      nbins += st[(16 + i) * 16];
+    }
+    ct0[6] = nbins;
+    e->info.nbinpath = (int64_t)nwalk;
     e->ran_candidates = false;
     if (fl0[3] || ct0[6]) {
       // cells over the walk budget or overflowing the fast path: per-bin

@@ -152,6 +152,11 @@ struct EvalBuffers {
   DBuf<unsigned long long> counters;  // [0] fragments, [1] candidates, [2] max poly, [3] empty,
                                       // [4] vertices, [5] FP64 re-decisions, [6] bin-path cells,
                                       // [7] failing cell (k_cell_huge), [8] long rows
+  // per-wave event counts of the cell kernel, striped over 16 cache lines
+  // (kStripes counters of 16 words each) so that no single address takes
+  // an atomic from every wave: [0..15] cells that walked the pyramid,
+  // [16..31] cells left to the bin path (word 0 of each line used)
+  DBuf<unsigned long long> stripes;
   DBuf<int32_t> scan_tmp;
   DBuf<double> red_tmp;
   // register/LDS fast path
