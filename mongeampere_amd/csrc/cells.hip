@@ -283,8 +283,11 @@ __device__ inline void consider(const CellArgs &a, int64_t k, int32_t j, double 
     if (wj > wk || (wj == wk && a.perm[j] < a.perm[k])) n = 0;
     return;
   }
-  const double d = sqrt(d2), rho = sqrt(rho2);
-  if (d2 - 2.0 * rho * d + wk - wj > 1e-12 * (d2 + fabs(wk) + fabs(wj))) return;
+  // j cannot cut the polygon (radius rho about the Dirac) if
+  // d2 - 2 rho d + wk - wj > tol; with c = d2 + wk - wj that is
+  // c - tol > 2 rho d, decided on squares (no square roots)
+  const double ct = d2 + wk - wj - 1e-12 * (d2 + fabs(wk) + fabs(wj));
+  if (ct > 0.0 && ct * ct > 4.0 * rho2 * d2) return;
   for (int v = 0; v < n; v++)
     if (L.sl[v][L.t] == j) return;  // already an edge of the cell
   HalfPlane<R> H;
