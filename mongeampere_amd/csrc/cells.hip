@@ -82,6 +82,7 @@ struct CellArgs {
   int force_general;  // route every cell to the general kernel (testing)
   int walk_budget;    // pyramid nodes a cell may visit before the bin path
   unsigned long long *stripes;  // striped event counters (EvalBuffers::stripes)
+  int32_t *ptri;      // first kPadTri candidate triangles of each cell, [c * n + k]
   FragOut fo;         // fragment polygons (mode 1, when fo.count is set)
 };
 
@@ -330,9 +331,13 @@ __device__ inline void write_cell(const CellArgs &a, int64_t k, double yx, doubl
   a.bbox[4 * k + 1] = by0;
   a.bbox[4 * k + 2] = bx1;
   a.bbox[4 * k + 3] = by1;
+  // candidate triangles: counted, and the first kPadTri kept for k_pairs
   int cnt = 0;
   for_each_triangle(a.g, a.tbin_start, a.tbin_items, a.tbox, bx0, by0, bx1, by1, 0, 0,
-                    a.g.G - 1, a.g.G - 1, [&](int32_t) { cnt++; });
+                    a.g.G - 1, a.g.G - 1, [&](int32_t ti) {
+                      if (cnt < kPadTri) a.ptri[cnt * N + k] = ti;
+                      cnt++;
+                    });
   a.tcount[k] = cnt;
 }
 
@@ -603,18 +608,47 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_bins(CellArgs a, const int3
   write_cell(a, k, yx, yy, L, n);
 }
 
-__global__ void k_pairs(CellArgs a, const int32_t *poff, int32_t *pair_cell, int32_t *pair_tri) {
-  const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (k >= a.n) return;
-  if (a.tcount[k] == 0) return;
-  int32_t o = poff[k];
-  for_each_triangle(a.g, a.tbin_start, a.tbin_items, a.tbox, a.bbox[4 * k], a.bbox[4 * k + 1],
-                    a.bbox[4 * k + 2], a.bbox[4 * k + 3], 0, 0, a.g.G - 1, a.g.G - 1,
-                    [&](int32_t ti) {
-                      pair_cell[o] = (int32_t)k;
-                      pair_tri[o] = ti;
-                      o++;
-                    });
+// One wave (one block) per 64 consecutive cells.  Their pair segments are
+// contiguous in the output, so the lanes write the wave's whole range
+// together (coalesced): slot s belongs to the last cell whose offset is <= s
+// (binary search over the 64 offsets in LDS) and takes that cell's kept
+// triangle.  Cells with more than kPadTri candidates were not kept by the
+// cell kernel; their own lane scans the triangle bins again.
+__global__ void __launch_bounds__(64) k_pairs(CellArgs a, const int32_t *poff, int32_t *pair_cell,
+                                              int32_t *pair_tri) {
+  __shared__ int32_t soff[65];
+  __shared__ int32_t skeep[64];
+  const int64_t N = a.n;
+  const int64_t k0 = blockIdx.x * (int64_t)64;
+  const int lane = (int)threadIdx.x;
+  const int64_t k = k0 + lane;
+  const int nk = (int)min((int64_t)64, N - k0);
+  const int cnt = lane < nk ? a.tcount[k] : 0;
+  soff[lane] = lane < nk ? poff[k] : poff[k0 + nk];
+  if (lane == 0) soff[64] = poff[k0 + nk];
+  skeep[lane] = cnt <= kPadTri;
+  __syncthreads();
+  if (cnt > kPadTri) {
+    int32_t o = soff[lane];
+    for_each_triangle(a.g, a.tbin_start, a.tbin_items, a.tbox, a.bbox[4 * k], a.bbox[4 * k + 1],
+                      a.bbox[4 * k + 2], a.bbox[4 * k + 3], 0, 0, a.g.G - 1, a.g.G - 1,
+                      [&](int32_t ti) {
+                        pair_cell[o] = (int32_t)k;
+                        pair_tri[o] = ti;
+                        o++;
+                      });
+  }
+  const int32_t lo = soff[0], hi = soff[64];
+  for (int32_t slot = lo + lane; slot < hi; slot += 64) {
+    int j = 0;  // last cell with soff[j] <= slot (offsets are non-decreasing)
+#pragma unroll
+    for (int step = 32; step > 0; step >>= 1)
+      if (j + step < nk && soff[j + step] <= slot) j += step;
+    if (!skeep[j]) continue;
+    const int c = slot - soff[j];
+    pair_cell[slot] = (int32_t)(k0 + j);
+    pair_tri[slot] = a.ptri[c * N + k0 + j];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1331,6 +1365,8 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   a.counters = B.counters.p;
   a.force_general = e->force_general;
   a.walk_budget = e->walk_budget;
+  MA_HIP_TRY(B.ptri.ensure((size_t)kPadTri * n));
+  a.ptri = B.ptri.p;
   MA_HIP_TRY(B.stripes.ensure(32 * 16));
   MA_HIP_TRY(hipMemsetAsync(B.stripes.p, 0, sizeof(unsigned long long) * 32 * 16, e->stream));
   a.stripes = B.stripes.p;
@@ -1400,7 +1436,7 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   MA_HIP_TRY(hipStreamSynchronize(e->stream));
   MA_HIP_TRY(B.pair_cell.ensure((size_t)np + 1));
   MA_HIP_TRY(B.pair_tri.ensure((size_t)np + 1));
-  hipLaunchKernelGGL(k_pairs, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a, B.poff.p,
+  hipLaunchKernelGGL(k_pairs, dim3(nblocks(n, 64)), dim3(64), 0, e->stream, a, B.poff.p,
                      B.pair_cell.p, B.pair_tri.p);
   MA_HIP_TRY(hipEventRecord(e->ev[4], e->stream));
   if (np > 0) {

@@ -95,6 +95,9 @@ constexpr int kLongRows = 1024;  // pool of staged rows longer than kMaxRow
 // (defaults; MA_OPT_LOCAL_RINGS / MA_OPT_WALK_BUDGET).
 constexpr int kLocalRings = 8;
 constexpr int kWalkBudget = 512;
+// Candidate triangles per cell kept by the cell kernels for the pair list
+// (cells with more are scanned again).
+constexpr int kPadTri = 24;
 
 // Fragment polygons written out by the first-moment kernels while `count` is
 // set (ma_fragments).  Each fragment of positive area reserves one record and
@@ -163,6 +166,7 @@ struct EvalBuffers {
   DBuf<double> px, py, eacc, bbox;
   DBuf<float> pxf, pyf, eaccf;   // FP32 polygons and edge integrals (mixed mode)
   DBuf<int32_t> plab, pn, tcount, poff, pair_cell, pair_tri, glist;
+  DBuf<int32_t> ptri;         // kPadTri candidate triangles per cell, [c * n + k]
   DBuf<int32_t> ovf;          // pairs for k_frag_wide (+ count)
   // cells beyond the general kernel (list + count at [n]) and their long rows
   DBuf<int32_t> hlist, hrow_col;
