@@ -305,3 +305,32 @@ def test_cell_polygons_lattice_and_large(engine):
     m, _ = engine.first_moment(np.zeros(100000))
     # (shoelace areas in absolute coordinates carry ~1e-16 of rounding)
     assert np.abs(areas - m).max() <= 1e-11 * m.max()
+
+
+def test_output_entry_points_reject_bad_input(engine):
+    from mongeampere_amd._lib import EngineError
+
+    # a non-convex polygon, a repeated vertex and a degenerate box are refused
+    arrow = np.array([[0.0, 0.0], [1.0, 0.0], [0.3, 0.3], [0.0, 1.0]])
+    with pytest.raises(EngineError):
+        engine.set_mesh_polygon(arrow)
+    with pytest.raises(EngineError):
+        engine.set_mesh_polygon(np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 0.0], [0.0, 1.0]]))
+    mesh, X, w = _random_case(8, 30, 6)
+    engine.set_mesh(mesh).set_diracs(X)
+    with pytest.raises(EngineError):
+        engine.draw_laguerre_diagram(w, np.ones(30), (1.0, 0.0, 0.0, 1.0), 8, 8)
+    with pytest.raises(ValueError):
+        engine.draw_laguerre_diagram(w, np.ones(29), (0.0, 0.0, 1.0, 1.0), 8, 8)
+    with pytest.raises(EngineError):
+        engine.set_mesh_image(np.ones((1, 5)))
+    # a Dirac hidden by a heavier coincident one: no fragments, no polygon
+    X2 = np.vstack([X, X[:1]])
+    w2 = np.append(w, w[0] + 1e-3)
+    engine.set_diracs(X2)
+    cell, tri, start, xy = engine.fragments(w2)
+    assert 0 not in set(cell.tolist()) and 30 in set(cell.tolist())
+    P = np.array([[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 1.0]])
+    engine.set_mesh_polygon(P).set_diracs(X2)
+    s, q = engine.cell_polygons(w2)
+    assert s[1] == s[0] and s[31] > s[30]
