@@ -29,7 +29,6 @@
 
 namespace ma {
 
-constexpr int kMaxFast = 16;
 constexpr int kCellBlock = 128;
 constexpr int kNodeDiracs = 8;    // pyramid nodes with at most this many Diracs are taken whole
 constexpr double kBoxBins = 8.0;  // FP32 starting box half-width, in bins
@@ -1202,9 +1201,15 @@ __global__ void __launch_bounds__(256) k_frag_wide(CellArgs a, const int32_t *__
 
 // Hessian row of every fast-path cell (include/MA/kantorovich.hpp:109-122):
 // H_kj = -int_{e_kj} rho ds / (2 |y_k - y_j|), H_kk = -sum_j H_kj, in FP64.
+// The row is sorted in LDS (per-thread columns [r][t]; private arrays indexed
+// at run time would live in scratch memory) and staged slot-major,
+// fcol[r * n + k], so that k_csr_fill reads it coalesced.
 template <class R>
-__global__ void k_rows(CellArgs a, double *fval_part, int32_t *row_cnt, int32_t *row_col,
-                       double *row_val, int cap_row) {
+__global__ void __launch_bounds__(kRowBlock) k_rows(CellArgs a, double *fval_part, int32_t *row_cnt,
+                                                    int32_t *fcol, double *fval) {
+  __shared__ int32_t lcol[kMaxFast + 1][kRowBlock];
+  __shared__ double lval[kMaxFast + 1][kRowBlock];
+  const int t = (int)threadIdx.x;
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int n = (k < a.n) ? a.pn[k] : -1;  // n < 0: the general kernel writes this row
   unsigned long long my_max = 0, my_empty = 0, my_nv = 0;
@@ -1212,8 +1217,8 @@ __global__ void k_rows(CellArgs a, double *fval_part, int32_t *row_cnt, int32_t 
     const int64_t N = a.n;
     const R *eacc = CellBuf<R>::eacc(a);
     const double yx = a.pos[2 * k], yy = a.pos[2 * k + 1];
-    int32_t cols[kMaxFast + 1];
-    double vals[kMaxFast + 1];
+    auto cols = [&](int r) -> int32_t & { return lcol[r][t]; };
+    auto vals = [&](int r) -> double & { return lval[r][t]; };
     int nr = 0;
     double diag = 0.0;
     for (int v = 0; v < n; v++) {
@@ -1223,36 +1228,37 @@ __global__ void k_rows(CellArgs a, double *fval_part, int32_t *row_cnt, int32_t 
       const double val = (double)eacc[v * N + k] / (2.0 * sqrt(dx * dx + dy * dy));
       diag += val;
       int p = 0;
-      while (p < nr && cols[p] < j) p++;
-      if (p < nr && cols[p] == j) {
-        vals[p] -= val;
+      while (p < nr && cols(p) < j) p++;
+      if (p < nr && cols(p) == j) {
+        vals(p) -= val;
         continue;
       }
       for (int r = nr; r > p; r--) {
-        cols[r] = cols[r - 1];
-        vals[r] = vals[r - 1];
+        cols(r) = cols(r - 1);
+        vals(r) = vals(r - 1);
       }
-      cols[p] = j;
-      vals[p] = -val;
+      cols(p) = j;
+      vals(p) = -val;
       nr++;
     }
     int cnt = 0;
     if (nr > 0) {
-      int32_t *oc = row_col + k * (int64_t)cap_row;
-      double *ov = row_val + k * (int64_t)cap_row;
       bool placed = false;
       for (int r = 0; r < nr; r++) {
-        if (!placed && cols[r] > k) {
-          oc[cnt] = (int32_t)k;
-          ov[cnt++] = diag;
+        if (!placed && cols(r) > k) {
+          fcol[cnt * N + k] = (int32_t)k;
+          fval[cnt * N + k] = diag;
+          cnt++;
           placed = true;
         }
-        oc[cnt] = cols[r];
-        ov[cnt++] = vals[r];
+        fcol[cnt * N + k] = cols(r);
+        fval[cnt * N + k] = vals(r);
+        cnt++;
       }
       if (!placed) {
-        oc[cnt] = (int32_t)k;
-        ov[cnt++] = diag;
+        fcol[cnt * N + k] = (int32_t)k;
+        fval[cnt * N + k] = diag;
+        cnt++;
       }
     }
     row_cnt[k] = cnt;
@@ -1466,11 +1472,11 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   }
   MA_HIP_TRY(hipEventRecord(e->ev[5], e->stream));
   if (mode == 0 && fp32)
-    hipLaunchKernelGGL(k_rows<float>, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a,
-                       B.fval_part.p, B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row);
+    hipLaunchKernelGGL(k_rows<float>, dim3(nblocks(n, kRowBlock)), dim3(kRowBlock), 0, e->stream, a,
+                       B.fval_part.p, B.row_cnt.p, B.frow_col.p, B.frow_val.p);
   else if (mode == 0)
-    hipLaunchKernelGGL(k_rows<double>, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a,
-                       B.fval_part.p, B.row_cnt.p, B.row_col.p, B.row_val.p, e->cap_row);
+    hipLaunchKernelGGL(k_rows<double>, dim3(nblocks(n, kRowBlock)), dim3(kRowBlock), 0, e->stream, a,
+                       B.fval_part.p, B.row_cnt.p, B.frow_col.p, B.frow_val.p);
   else if (mode == 1 || mode == 3)  // mode 2 keeps the moments about the Dirac
     hipLaunchKernelGGL(k_mom_finalize, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, a,
                        (int)(mode == 3));

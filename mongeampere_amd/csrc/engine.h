@@ -98,6 +98,9 @@ constexpr int kWalkBudget = 512;
 // Candidate triangles per cell kept by the cell kernels for the pair list
 // (cells with more are scanned again).
 constexpr int kPadTri = 24;
+// Polygon capacity of the register/LDS fast path (vertices per cell).
+constexpr int kMaxFast = 16;
+constexpr int kRowBlock = 128;  // threads per block of k_rows (its LDS rows)
 
 // Fragment polygons written out by the first-moment kernels while `count` is
 // set (ma_fragments).  Each fragment of positive area reserves one record and
@@ -146,8 +149,10 @@ struct EvalBuffers {
   DBuf<double> mom2_s;       // second moments (3 per Dirac, mode 3)
   DBuf<double> dval;         // adjoint w.r.t. vertex density values (nv)
   DBuf<int32_t> row_cnt;     // per Dirac, entries incl. diagonal
-  DBuf<int32_t> row_col;     // n x cap_row
+  DBuf<int32_t> row_col;     // n x cap_row (rows of the general kernels)
   DBuf<double> row_val;      // n x cap_row
+  DBuf<int32_t> frow_col;    // fast-path rows, slot-major [r * n + k], r <= kMaxFast
+  DBuf<double> frow_val;
   DBuf<int32_t> csr_ptr, csr_col;   // Morton-order CSR
   DBuf<double> csr_val;
   DBuf<double> diag;

@@ -1515,14 +1515,30 @@ __global__ void __launch_bounds__(kHugeBlock) k_cell_huge(
 
 // Rows longer than cap_row (from k_cell_huge) are read from the long-row
 // staging at the slot stored in their first regular entry.
-__global__ void k_csr_fill(int64_t n, const int32_t *row_cnt, const int32_t *row_col,
-                           const double *row_val, int cap_row, const int32_t *hrow_col,
-                           const double *hrow_val, const int32_t *ptr, int32_t *col,
-                           double *val, double *diag) {
+// CSR rows from the staged rows: fast-path cells (pn >= 0) slot-major in
+// frow_*, rows of the general kernels per Dirac in row_* (or, past cap_row
+// entries, in the long-row pool hrow_*).
+__global__ void k_csr_fill(int64_t n, const int32_t *pn, const int32_t *row_cnt,
+                           const int32_t *frow_col, const double *frow_val,
+                           const int32_t *row_col, const double *row_val, int cap_row,
+                           const int32_t *hrow_col, const double *hrow_val, const int32_t *ptr,
+                           int32_t *col, double *val, double *diag) {
   const int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int c = row_cnt[k];
   const int32_t o = ptr[k];
+  double d = 0.0;
+  if (pn[k] >= 0) {
+    for (int i = 0; i < c; i++) {
+      const int32_t j = frow_col[i * n + k];
+      const double v = frow_val[i * n + k];
+      col[o + i] = j;
+      val[o + i] = v;
+      if (j == k) d = v;
+    }
+    diag[k] = d;
+    return;
+  }
   const int32_t *rc = row_col + k * (int64_t)cap_row;
   const double *rv = row_val + k * (int64_t)cap_row;
   if (c > cap_row) {
@@ -1530,7 +1546,6 @@ __global__ void k_csr_fill(int64_t n, const int32_t *row_cnt, const int32_t *row
     rc = hrow_col + s * kHugeRow;
     rv = hrow_val + s * kHugeRow;
   }
-  double d = 0.0;
   for (int i = 0; i < c; i++) {
     const int32_t j = rc[i];
     const double v = rv[i];
@@ -1636,6 +1651,8 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   MA_HIP_TRY(B.row_cnt.ensure((size_t)n + 1));
   MA_HIP_TRY(B.row_col.ensure((size_t)n * e->cap_row));
   MA_HIP_TRY(B.row_val.ensure((size_t)n * e->cap_row));
+  MA_HIP_TRY(B.frow_col.ensure((size_t)n * (kMaxFast + 1)));
+  MA_HIP_TRY(B.frow_val.ensure((size_t)n * (kMaxFast + 1)));
   MA_HIP_TRY(B.flags.ensure(4));
   MA_HIP_TRY(B.counters.ensure(12));
   MA_HIP_TRY(B.diag.ensure((size_t)n));
@@ -1718,9 +1735,10 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
     MA_HIP_TRY(hipStreamSynchronize(e->stream));
     MA_HIP_TRY(B.csr_col.ensure((size_t)nnz + 1));
     MA_HIP_TRY(B.csr_val.ensure((size_t)nnz + 1));
-    hipLaunchKernelGGL(k_csr_fill, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, n, B.row_cnt.p,
-                       B.row_col.p, B.row_val.p, e->cap_row, B.hrow_col.p, B.hrow_val.p,
-                       B.csr_ptr.p, B.csr_col.p, B.csr_val.p, B.diag.p);
+    hipLaunchKernelGGL(k_csr_fill, dim3(nblocks(n)), dim3(kBlock), 0, e->stream, n, B.pn.p,
+                       B.row_cnt.p, B.frow_col.p, B.frow_val.p, B.row_col.p, B.row_val.p,
+                       e->cap_row, B.hrow_col.p, B.hrow_val.p, B.csr_ptr.p, B.csr_col.p,
+                       B.csr_val.p, B.diag.p);
     MA_HIP_TRY(device_sum(e, B.fval_part.p, n, B.red_tmp.p + 1024));
     e->nnz = nnz;
   }
