@@ -444,9 +444,12 @@ __global__ void __launch_bounds__(kCellBlock) k_cell_local(CellArgs a, const dou
   int r = 0;
   for (; r <= rmax && n > 0 && !fail; r++) {
     for (int by = max(sby - r, 0); by <= min(sby + r, g.G - 1) && n > 0 && !fail; by++) {
+      // the ring's bins in this row: all of them on its first and last row,
+      // otherwise only the two ends (stepping straight from one to the other)
       const bool edge_row = (by == sby - r) || (by == sby + r);
-      for (int bx = max(sbx - r, 0); bx <= min(sbx + r, g.G - 1) && n > 0 && !fail; bx++) {
-        if (!edge_row && bx != sbx - r && bx != sbx + r) continue;
+      const int bstep = (edge_row || r == 0) ? 1 : 2 * r;
+      for (int bx = sbx - r; bx <= sbx + r && n > 0 && !fail; bx += bstep) {
+        if (bx < 0 || bx >= g.G) continue;
         const int64_t b = morton2((uint32_t)bx, (uint32_t)by);
         for (int32_t q = a.bin_start[b]; q < a.bin_start[b + 1] && n > 0 && !fail; q++)
           consider(a, k, q, yx, yy, wk, L, n, rho2, fail, nredo);
