@@ -109,6 +109,25 @@ def test_random_vs_oracle(engine, seed, path):
         assert _pattern(H, 1e-9) == _pattern(oH, 1e-9)
 
 
+@pytest.mark.parametrize("N", [30, 300])
+def test_cells_over_many_triangles_vs_oracle(engine, N):
+    """Few Diracs on a fine mesh: every cell meets far more triangles than the
+    cell kernels keep for the pair list (24), so the pairs come from a
+    second scan of the triangle bins."""
+    rng = np.random.Generator(np.random.PCG64(77 + N))
+    vxy, tri = configs.grid_mesh(64)
+    mesh = configs.Mesh(vxy, rng.uniform(0.5, 1.5, vxy.shape[0]), tri)
+    X = rng.uniform(0, 1, (N, 2))
+    w = rng.uniform(-0.2, 0.2, N) / N
+    engine.set_mesh(mesh).set_diracs(X)
+    fval, g, H = engine.kantorovich(w)
+    assert engine.eval_info()["npairs"] > 24 * N
+    of, og, oH = oracle.kantorovich(mesh, X, w)
+    assert _rel(g, og) <= TOL
+    assert abs(fval - of) <= TOL * abs(of)
+    assert _rel(H.toarray(), oH.toarray()) <= TOL
+
+
 @pytest.mark.parametrize("path", ["default", "general"])
 @pytest.mark.parametrize("nring", [59, 200, 450])
 def test_large_cell_vs_oracle(engine, nring, path):
