@@ -185,8 +185,9 @@ def main():
                     help="run one device copy of this many MiB first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this engine has no multi-GPU path "
-              "(see DESIGN.md for how the evaluation would shard).", file=sys.stderr)
+        print("bench.py runs on one GPU only: this engine has no multi-GPU path (the "
+              "evaluation would shard by Morton ranges of Diracs with a halo exchange).",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
