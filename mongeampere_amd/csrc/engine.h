@@ -224,6 +224,8 @@ struct ma_engine {
   int use_graphs = 1;     // MA_OPT_GRAPHS: replay CG batches from a HIP graph
   int local_rings = ma::kLocalRings;  // MA_OPT_LOCAL_RINGS
   int walk_budget = ma::kWalkBudget;  // MA_OPT_WALK_BUDGET
+  double amg_omega = 0.85;  // MA_OPT_AMG_OMEGA: Jacobi damping of the multigrid smoother
+  double amg_corr = 1.5;         // MA_OPT_AMG_CORR: coarse-correction scaling
   bool ran_candidates = false;  // the last evaluation built per-bin candidate lists
   int cap_row = ma::kMaxRow;
 

@@ -984,6 +984,14 @@ ma_status ma_set_option(ma_engine *e, int option, int64_t value) {
       if (value < 0 || value > (1 << 20)) return MA_BAD_INPUT;
       e->walk_budget = (int)value;
       return MA_OK;
+    case MA_OPT_AMG_OMEGA:
+      if (value < 100 || value > 1000) return MA_BAD_INPUT;
+      e->amg_omega = value * 1e-3;
+      return MA_OK;
+    case MA_OPT_AMG_CORR:
+      if (value < 500 || value > 3000) return MA_BAD_INPUT;
+      e->amg_corr = value * 1e-3;
+      return MA_OK;
     default:
       return MA_BAD_INPUT;
   }
