@@ -314,6 +314,7 @@ ma_status amg_setup(ma_engine *e, Amg &A, const CsrView &M, int32_t gi, int max_
 hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const double *done) {
   const int nl = A.nlev;
   const double om = e->amg_omega, co = e->amg_corr;  // Jacobi damping, coarse over-correction
+  const bool two = e->amg_sweeps != 1;                 // V(2,2), else V(1,1)
   auto ptr_of = [&](int l) { return l == 0 ? A.ptr0 : A.lv[l].ptr.p; };
   auto col_of = [&](int l) { return l == 0 ? A.col0 : A.lv[l].col.p; };
   // downward (the first sweep of a coarse level comes from the restriction)
@@ -324,8 +325,11 @@ hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const dou
     if (l == 0)
       hipLaunchKernelGGL(k_amg_smooth0, dim3(nb2(L.n)), dim3(kVB2), 0, e->stream, L.n, L.dinv.p,
                          b, L.y.p, done, om);
-    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
-                       ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done, om);
+    if (two)
+      hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
+                         ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done, om);
+    else
+      x = L.y.p;  // V(1,1): the one downward sweep is the iterate
     AmgLevel &Ln = A.lv[l + 1];
     const bool seed = l + 1 < nl - 1;  // the coarsest level is solved directly
     hipLaunchKernelGGL(k_amg_restrict, dim3(nb2(Ln.n * kAgg * kRowLanes)), dim3(kVB2), 0,
@@ -345,11 +349,18 @@ hipError_t amg_apply(ma_engine *e, Amg &A, const double *r, double *z, const dou
     AmgLevel &L = A.lv[l];
     const double *b = (l == 0) ? r : L.b.p;
     double *x = (l == 0) ? z : L.x.p;
-    hipLaunchKernelGGL(k_amg_smooth_prolong, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0,
-                       e->stream, L.n, ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, x,
-                       A.lv[l + 1].x.p, L.y.p, done, om, co);
-    hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
-                       ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done, om);
+    if (two) {
+      hipLaunchKernelGGL(k_amg_smooth_prolong, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0,
+                         e->stream, L.n, ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, x,
+                         A.lv[l + 1].x.p, L.y.p, done, om, co);
+      hipLaunchKernelGGL(k_amg_smooth, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0, e->stream, L.n,
+                         ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p, x, done, om);
+    } else {
+      // V(1,1): the downward iterate is in y; one sweep after the correction
+      hipLaunchKernelGGL(k_amg_smooth_prolong, dim3(nb2(L.n * kRowLanes)), dim3(kVB2), 0,
+                         e->stream, L.n, ptr_of(l), col_of(l), L.val.p, L.dinv.p, b, L.y.p,
+                         A.lv[l + 1].x.p, x, done, om, co);
+    }
   }
   return hipGetLastError();
 }

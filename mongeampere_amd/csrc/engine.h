@@ -226,6 +226,7 @@ struct ma_engine {
   int walk_budget = ma::kWalkBudget;  // MA_OPT_WALK_BUDGET
   double amg_omega = 0.85;  // MA_OPT_AMG_OMEGA: Jacobi damping of the multigrid smoother
   double amg_corr = 1.5;         // MA_OPT_AMG_CORR: coarse-correction scaling
+  int amg_sweeps = 2;            // MA_OPT_AMG_SWEEPS: smoothing sweeps each way (1 or 2)
   bool ran_candidates = false;  // the last evaluation built per-bin candidate lists
   int cap_row = ma::kMaxRow;
 

@@ -992,6 +992,10 @@ ma_status ma_set_option(ma_engine *e, int option, int64_t value) {
       if (value < 500 || value > 3000) return MA_BAD_INPUT;
       e->amg_corr = value * 1e-3;
       return MA_OK;
+    case MA_OPT_AMG_SWEEPS:
+      if (value != 1 && value != 2) return MA_BAD_INPUT;
+      e->amg_sweeps = (int)value;
+      return MA_OK;
     default:
       return MA_BAD_INPUT;
   }
