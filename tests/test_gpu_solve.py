@@ -119,3 +119,31 @@ def test_engine_reuse_across_sizes():
             assert s1.status == s2.status == "MA_OK"
             assert np.max(np.abs(w1 - w2)) <= 1e-9 * np.max(np.abs(w2)) + 1e-15
     shared.close()
+
+
+def test_multigrid_cycle_options(engine):
+    """V(1,1) and V(2,2) cycles with other damping converge to the same
+    weights (the preconditioner changes the path, not the solution); values
+    outside the documented ranges are refused."""
+    from mongeampere_amd._lib import EngineError
+
+    p = configs.config2(N=20000, n=128)
+    engine.set_mesh(p.mesh).set_diracs(p.X)
+    ws = []
+    try:
+        for sweeps, omega in ((2, 850), (1, 850), (2, 667)):
+            engine.set_option("amg_sweeps", sweeps)
+            engine.set_option("amg_omega_permille", omega)
+            w, st = engine.ot_solve(p.nu, stop="maxrel", tol=1e-9)
+            assert st.status == "MA_OK" and st.max_rel_err <= 1e-9
+            ws.append(w - w[-1])
+    finally:
+        engine.set_option("amg_sweeps", 2)
+        engine.set_option("amg_omega_permille", 850)
+    # the mass tolerance 1e-9 leaves the weights free at the level of the
+    # Laplacian's conditioning times 1e-9
+    for w in ws[1:]:
+        assert np.abs(w - ws[0]).max() <= 1e-4 * np.abs(ws[0]).max()
+    for name, bad in (("amg_sweeps", 3), ("amg_omega_permille", 50), ("amg_corr_permille", 4000)):
+        with pytest.raises(EngineError):
+            engine.set_option(name, bad)
