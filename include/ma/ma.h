@@ -97,6 +97,8 @@ typedef struct {
                            over budget)                                       */
   int64_t nhuge;        /* cells beyond the general kernel's 60 vertices,
                            evaluated one workgroup per cell                   */
+  double ms_cell_kernel; /* the site-centric cell kernel alone (inside
+                           ms_cell_build)                                     */
 } ma_eval_info;
 
 ma_status ma_create(int device, ma_engine **out);

@@ -42,7 +42,8 @@ class EvalInfo(ctypes.Structure):
                 ("ms_pairs", F64), ("ms_frag", F64), ("nnz", I64), ("nfrag", I64),
                 ("ncand", I64), ("max_poly", I64), ("empty_cells", I64), ("npairs", I64),
                 ("ngeneral", I64), ("nbinpath", I64), ("nvert", I64),
-                ("nredecide", I64), ("nbincand", I64), ("nhuge", I64)]
+                ("nredecide", I64), ("nbincand", I64), ("nhuge", I64),
+                ("ms_cell_kernel", F64)]
 
 
 # (name, restype, argtypes) of every exported entry point of include/ma/ma.h

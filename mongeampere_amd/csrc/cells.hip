@@ -1388,12 +1388,14 @@ ma_status run_cells(ma_engine *e, int mode, bool fp32, int32_t *npairs_out,
   const int32_t zero = 0;
   PyrOff po{};
   for (int l = 0; l <= e->grid.L; l++) po.o[l] = e->pyr_off[l];
+  MA_HIP_TRY(hipEventRecord(e->ev[10], e->stream));
   if (fp32)
     hipLaunchKernelGGL(k_cell_local<float>, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0,
                        e->stream, a, B.pyr.p, B.pyr3.p, po, e->local_rings);
   else
     hipLaunchKernelGGL(k_cell_local<double>, dim3(nblocks(n, kCellBlock)), dim3(kCellBlock), 0,
                        e->stream, a, B.pyr.p, B.pyr3.p, po, e->local_rings);
+  MA_HIP_TRY(hipEventRecord(e->ev[11], e->stream));
   {
     int32_t fl0[4];
     unsigned long long ct0[8], st[32 * 16];

@@ -191,7 +191,7 @@ struct SolverBuffers {
 struct ma_engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[10] = {};
+  hipEvent_t ev[12] = {};  // [10], [11]: around the site-centric cell kernel
   std::string err;
 
   // mesh

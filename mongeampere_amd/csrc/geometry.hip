@@ -1756,6 +1756,9 @@ ma_status evaluate(ma_engine *e, int mode, double *fval_out, bool fp32) {
   float t[7] = {0, 0, 0, 0, 0, 0, 0}, tc = 0.0f;
   for (int i = 0; i < 7; i++) MA_HIP_TRY(hipEventElapsedTime(&t[i], e->ev[i], e->ev[i + 1]));
   if (e->ran_candidates) MA_HIP_TRY(hipEventElapsedTime(&tc, e->ev[8], e->ev[9]));
+  float tk = 0.0f;
+  MA_HIP_TRY(hipEventElapsedTime(&tk, e->ev[10], e->ev[11]));
+  e->info.ms_cell_kernel = tk;
   e->info.ms_weights = t[0];
   e->info.ms_candidates = tc;
   e->info.ms_cell_build = t[2];
