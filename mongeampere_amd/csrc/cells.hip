@@ -29,7 +29,7 @@
 
 namespace ma {
 
-constexpr int kCellBlock = 128;
+constexpr int kCellBlock = 64;
 constexpr int kNodeDiracs = 8;    // pyramid nodes with at most this many Diracs are taken whole
 constexpr double kBoxBins = 8.0;  // FP32 starting box half-width, in bins
 constexpr int kBoxLabel = -3;     // edge label of that box's cut
